@@ -1,0 +1,207 @@
+"""culof_amd - Local Outlier Factor on an AMD Instinct MI355X.
+
+Python surface of the reference package (src/culof/__init__.py:31): the
+estimator :class:`LOF` (``fit``, ``fit_predict``, ``score_samples``,
+``predict``, ``negative_outlier_factor_``, ``offset_``, ``n_neighbors_``, with
+scikit-learn's sign conventions) and the function :func:`lof`. It covers the
+transductive fit_predict workflow and is not a full scikit-learn estimator.
+
+    >>> from culof_amd import LOF, lof
+    >>> labels = LOF(n_neighbors=20).fit_predict(X)   # -1 outlier, 1 inlier
+    >>> scores = lof(X, k=20)                         # ~1.0 normal, higher = more anomalous
+
+Every call runs the HIP kernels of ``libculof_amd.so`` through the ``_culof``
+extension; there is no CPU fallback. Importing fails if the native library has
+not been built (``python -m culof_amd.build``).
+"""
+
+from __future__ import annotations
+
+import math
+from typing import TYPE_CHECKING
+
+import numpy as np
+
+try:
+    from . import _culof
+except ImportError as exc:  # pragma: no cover - exercised only on unbuilt trees
+    raise ImportError(
+        "culof_amd's native extension is not built; run `python -m culof_amd.build` "
+        f"(original error: {exc})"
+    ) from exc
+
+from ._version import __version__
+
+if TYPE_CHECKING:  # pragma: no cover
+    from numpy.typing import ArrayLike, NDArray
+
+__all__ = ["LOF", "CudaError", "__version__", "cuda_available", "device_info", "lof"]
+
+cuda_available = _culof.cuda_available
+device_info = _culof.device_info
+CudaError = _culof.CudaError
+
+
+def _validate(X: ArrayLike, k: int) -> NDArray[np.float32]:
+    """Coerce and check the input (reference: src/culof/__init__.py:37-55)."""
+    arr = np.ascontiguousarray(X, dtype=np.float32)
+    if arr.ndim != 2:
+        raise ValueError(
+            f"expected a 2-D array of shape (n_samples, n_features), got shape {arr.shape}"
+        )
+    n_samples, n_features = arr.shape
+    if n_features < 1:
+        raise ValueError("need at least one feature")
+    if k < 1:
+        raise ValueError(f"n_neighbors must be >= 1, got {k}")
+    if k >= n_samples:
+        raise ValueError(f"n_neighbors={k} requires at least {k + 1} samples, got {n_samples}")
+    if not np.isfinite(arr).all():
+        raise ValueError("input contains NaN or infinity")
+    _check_representable(arr, n_features)
+    return arr
+
+
+def _check_representable(arr: NDArray[np.float32], n_features: int) -> None:
+    """Reject input whose squared distances would overflow float32.
+
+    After centring a coordinate is at most twice the input's largest magnitude,
+    so a squared distance is at most ``n_features * (4 * max|x|)**2``. Beyond
+    FLT_MAX every key would saturate to +inf and every score would be NaN,
+    which the estimator would read as "inlier". The bound is compared in logs
+    so the check cannot overflow itself (reference: src/culof/__init__.py:58-80).
+    """
+    if arr.size == 0:
+        return
+    max_abs = float(np.max(np.abs(arr)))
+    if max_abs == 0.0:
+        return
+    log_bound = math.log(n_features) + 2.0 * math.log(4.0 * max_abs)
+    if log_bound > math.log(float(np.finfo(np.float32).max)):
+        raise ValueError(
+            f"input magnitude {max_abs:.3g} overflows float32 when squared "
+            f"({n_features} features): distances would saturate to infinity and "
+            f"every score would be NaN. Rescale the data, or pass normalize=True "
+            f"to z-score each feature first."
+        )
+
+
+def lof(X: ArrayLike, k: int = 20, normalize: bool = False) -> NDArray[np.float32]:
+    """Local Outlier Factor of every row of ``X`` (reference: src/culof/__init__.py:83-113).
+
+    Parameters
+    ----------
+    X : array-like of shape (n_samples, n_features)
+        Converted to C-contiguous float32 if it is not already.
+    k : int, default=20
+        Neighbours per point, excluding the point itself; ``1 <= k <= n_samples - 1``.
+    normalize : bool, default=False
+        Z-score each feature before computing distances. scikit-learn does not,
+        so leave it off when comparing.
+
+    Returns
+    -------
+    ndarray of shape (n_samples,), float32
+        About 1.0 for a point as densely surrounded as its neighbours, larger
+        for one in a comparatively sparse region.
+    """
+    scores = _culof.lof(_validate(X, k), k, normalize)
+    # A non-finite score would read as "not an outlier" downstream.
+    if not np.isfinite(scores).all():
+        raise ValueError(
+            "LOF produced non-finite scores; this indicates float32 overflow or "
+            "underflow in the distance computation. Rescale the data, or pass "
+            "normalize=True."
+        )
+    return scores
+
+
+class LOF:
+    """Unsupervised outlier detection with Local Outlier Factor on the GPU.
+
+    The transductive ``fit_predict`` workflow of
+    :class:`sklearn.neighbors.LocalOutlierFactor` with the same sign
+    conventions (reference: src/culof/__init__.py:116-227). There is no
+    ``novelty`` mode, no ``get_params``/``set_params`` and no metric selection.
+
+    Parameters
+    ----------
+    n_neighbors : int, default=20
+        Neighbours used to measure local density; ``1 <= n_neighbors <= n_samples - 1``.
+    contamination : {'auto'} or float, default='auto'
+        Expected proportion of outliers, used to set :attr:`offset_`. ``'auto'``
+        uses scikit-learn's threshold of -1.5.
+    normalize : bool, default=False
+        Z-score each feature before computing distances.
+
+    Attributes
+    ----------
+    negative_outlier_factor_ : ndarray of shape (n_samples,)
+        Negated LOF of the fitted samples; the more negative, the more abnormal.
+    offset_ : float
+        Threshold on :attr:`negative_outlier_factor_` used by :meth:`predict`.
+    n_neighbors_ : int
+        The value of ``n_neighbors`` actually used.
+    """
+
+    def __init__(
+        self,
+        n_neighbors: int = 20,
+        contamination: str | float = "auto",
+        normalize: bool = False,
+    ) -> None:
+        if n_neighbors < 1:
+            raise ValueError(f"n_neighbors must be >= 1, got {n_neighbors}")
+        if contamination != "auto" and not 0.0 < float(contamination) <= 0.5:
+            raise ValueError(f"contamination must be 'auto' or in (0, 0.5], got {contamination!r}")
+        self.n_neighbors = n_neighbors
+        self.contamination = contamination
+        self.normalize = normalize
+
+    def fit(self, X: ArrayLike, y: object = None) -> LOF:
+        """Compute the LOF of every sample in ``X``. Returns ``self``."""
+        del y  # accepted for scikit-learn call compatibility
+        scores = lof(X, self.n_neighbors, self.normalize)
+        self.n_neighbors_ = self.n_neighbors
+        self.negative_outlier_factor_ = -scores
+        if self.contamination == "auto":
+            self.offset_ = -1.5
+        else:
+            self.offset_ = float(
+                np.percentile(self.negative_outlier_factor_, 100.0 * float(self.contamination))
+            )
+        return self
+
+    def fit_predict(self, X: ArrayLike, y: object = None) -> NDArray[np.intp]:
+        """Fit and label each sample: ``-1`` for outliers, ``1`` for inliers."""
+        del y
+        return self.fit(X).predict()
+
+    def score_samples(self) -> NDArray[np.float32]:
+        """Negated LOF of the fitted data; the lower, the more abnormal.
+
+        Takes no ``X``: scoring new points against each other would not be
+        scikit-learn's novelty mode, so the argument does not exist.
+        """
+        return self._check_fitted()
+
+    def predict(self) -> NDArray[np.intp]:
+        """Label the fitted samples ``-1`` (outlier) or ``1`` (inlier)."""
+        return np.where(self.score_samples() < self._check_offset(), -1, 1)
+
+    def _check_fitted(self) -> NDArray[np.float32]:
+        if not hasattr(self, "negative_outlier_factor_"):
+            raise ValueError(
+                "this LOF instance is not fitted yet - call fit() or fit_predict() first"
+            )
+        return self.negative_outlier_factor_
+
+    def _check_offset(self) -> float:
+        self._check_fitted()
+        return self.offset_
+
+    def __repr__(self) -> str:
+        return (
+            f"LOF(n_neighbors={self.n_neighbors}, "
+            f"contamination={self.contamination!r}, normalize={self.normalize})"
+        )

@@ -1,0 +1,478 @@
+// Host orchestration, the C ABI (include/culof_c.h) and the C++ API
+// (include/culof.h) of libculof_amd.
+//
+// The forward pass is the reference's culof::lof (src/cuda/pipeline.cu:131-207)
+// re-planned for one MI355X:
+//   centre + norms (prep.hip) -> fused sweep kNN for every query row
+//   (sweep.hip; no distance tile exists, so memory is O(n k)) -> exact radix
+//   redo of any row the sweep hands back (radix.hip; a fixed grid-stride
+//   launch, no host read-back) -> LRD -> score (density.hip).
+// Scratch is stream-ordered (hipMallocAsync on the call's stream), so calls on
+// different streams or threads never share memory and nothing synchronises
+// the device mid-pipeline. CULOF_TILE_ROWS still splits the query rows into
+// separate launches; results are bit-identical for any tile height because
+// every query row is computed independently.
+
+#include "internal.hpp"
+
+#include "culof.h"
+#include "culof_c.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace culof {
+namespace detail {
+
+void check_hip(hipError_t err, const char* file, int line, const char* expr) {
+    if (err == hipSuccess) return;
+    std::ostringstream os;
+    os << "HIP error at " << file << ":" << line << " (" << expr << "): " << hipGetErrorString(err);
+    throw CudaError(os.str());
+}
+
+int choose_tile_rows(int n_points, size_t budget_bytes) {
+    const size_t row_bytes = static_cast<size_t>(n_points) * sizeof(float);
+    size_t rows = budget_bytes / std::max<size_t>(row_bytes, 1);
+    rows = std::clamp<size_t>(rows, 1, static_cast<size_t>(std::max(n_points, 1)));
+    return static_cast<int>(rows);
+}
+
+// ---------------------------------------------------------------------------
+// Kernel timing registry (culof_profile_*).
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct ProfRecord {
+    std::string name;
+    hipEvent_t start = nullptr;
+    hipEvent_t stop = nullptr;
+};
+
+std::atomic<bool> g_prof_on{false};
+std::mutex g_prof_mu;
+std::vector<ProfRecord> g_prof;
+
+}  // namespace
+
+ProfileScope::ProfileScope(const char* name, hipStream_t stream) : stream_(stream) {
+    if (!g_prof_on.load(std::memory_order_relaxed)) return;
+    ProfRecord rec;
+    rec.name = name;
+    CULOF_CHECK(hipEventCreate(&rec.start));
+    CULOF_CHECK(hipEventCreate(&rec.stop));
+    CULOF_CHECK(hipEventRecord(rec.start, stream));
+    std::lock_guard<std::mutex> lock(g_prof_mu);
+    slot_ = static_cast<int>(g_prof.size());
+    g_prof.push_back(rec);
+}
+
+ProfileScope::~ProfileScope() {
+    if (slot_ < 0) return;
+    hipEvent_t stop;
+    {
+        std::lock_guard<std::mutex> lock(g_prof_mu);
+        if (slot_ >= static_cast<int>(g_prof.size())) return;  // reset meanwhile
+        stop = g_prof[slot_].stop;
+    }
+    (void)hipEventRecord(stop, stream_);
+}
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Stream-ordered scratch.
+// ---------------------------------------------------------------------------
+
+void enable_pool_reuse(int device) {
+    // Keep freed blocks in the device's default pool so repeated calls do not
+    // return memory to the driver and fault it back in.
+    static std::mutex mu;
+    static std::vector<int> done;
+    std::lock_guard<std::mutex> lock(mu);
+    if (std::find(done.begin(), done.end(), device) != done.end()) return;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+        uint64_t threshold = ~0ull;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+    }
+    done.push_back(device);
+}
+
+class Scratch {
+public:
+    explicit Scratch(hipStream_t s) : stream_(s) {}
+    ~Scratch() {
+        for (void* p : ptrs_) (void)hipFreeAsync(p, stream_);
+    }
+    Scratch(const Scratch&) = delete;
+    Scratch& operator=(const Scratch&) = delete;
+
+    template <typename T>
+    T* get(size_t count) {
+        void* p = nullptr;
+        CULOF_CHECK(hipMallocAsync(&p, std::max<size_t>(count * sizeof(T), 256), stream_));
+        ptrs_.push_back(p);
+        return static_cast<T*>(p);
+    }
+
+private:
+    hipStream_t stream_;
+    std::vector<void*> ptrs_;
+};
+
+class Stream {
+public:
+    Stream() { CULOF_CHECK(hipStreamCreate(&s_)); }
+    ~Stream() {
+        if (s_) (void)hipStreamDestroy(s_);
+    }
+    Stream(const Stream&) = delete;
+    Stream& operator=(const Stream&) = delete;
+    hipStream_t get() const { return s_; }
+
+private:
+    hipStream_t s_ = nullptr;
+};
+
+void validate(const void* points, int n_points, int n_dims, int k) {
+    if (points == nullptr) throw std::invalid_argument("points must not be null");
+    if (n_points < 2) throw std::invalid_argument("n_points must be >= 2");
+    if (n_dims < 1) throw std::invalid_argument("n_dims must be >= 1");
+    if (k < 1 || k >= n_points) {
+        std::ostringstream os;
+        os << "k must satisfy 1 <= k <= n_points - 1 (got k=" << k << ", n_points=" << n_points
+           << ")";
+        throw std::invalid_argument(os.str());
+    }
+}
+
+bool device_present() {
+    int count = 0;
+    return hipGetDeviceCount(&count) == hipSuccess && count > 0;
+}
+
+void require_device() {
+    if (!device_present()) throw CudaError("no HIP device available");
+}
+
+// Rows per sweep launch: CULOF_TILE_ROWS when set (the reference's knob,
+// src/cuda/pipeline.cu:99-103), else every row in one launch.
+int tile_rows_for(int n_points) {
+    if (const char* env = std::getenv("CULOF_TILE_ROWS")) {
+        const int forced = std::atoi(env);
+        if (forced > 0) return std::min(forced, n_points);
+    }
+    return n_points;
+}
+
+// Device-resident inputs of the selection stage, built by prepare().
+struct Prepared {
+    int stride = 0;             // padded row length shared by both row sets
+    const float* xs = nullptr;  // key rows: the input, or the z-scored rows
+    const float* xc = nullptr;  // centred (and scaled) rows for the Gram screen
+    const float* norms = nullptr;
+    const int* guard = nullptr;
+};
+
+Prepared prepare(const float* points, int n, int d, bool normalize, Scratch& scratch,
+                 hipStream_t s) {
+    Prepared p;
+    p.stride = padded_dims(d);
+    const size_t cells = static_cast<size_t>(n) * p.stride;
+    float* xc = scratch.get<float>(cells);
+    float* norms = scratch.get<float>(n);
+    double* work = scratch.get<double>(static_cast<size_t>(258) * d);
+    int* guard = scratch.get<int>(1);
+    float* xs = normalize ? nullptr : scratch.get<float>(cells);
+    CULOF_CHECK(hipMemsetAsync(guard, 0, sizeof(int), s));
+    // Centre always; scale only when asked (src/cuda/pipeline.cu:162-165).
+    center_and_norms(points, n, d, normalize, xc, p.stride, norms, work, xs, guard, s);
+    p.xs = normalize ? xc : xs;
+    p.xc = xc;
+    p.norms = norms;
+    p.guard = guard;
+    return p;
+}
+
+// kNN of query rows [row0, row0 + rows); method 0 = fused sweep with exact
+// radix redo of handed-back rows, 1 = radix for every row.
+void knn_rows(const Prepared& p, int n, int d, int k, int row0, int rows, int method,
+              int* nbr_idx, float* nbr_d2, float* kdist2, Scratch& scratch, hipStream_t s) {
+    const int tile = tile_rows_for(rows);
+    if (method == 0 && sweep_supported(d, k)) {
+        int* fb_count = scratch.get<int>(1);
+        int* fb_rows = scratch.get<int>(static_cast<size_t>(rows));
+        CULOF_CHECK(hipMemsetAsync(fb_count, 0, sizeof(int), s));
+        for (int r0 = 0; r0 < rows; r0 += tile) {
+            sweep_knn(p.xs, p.xc, p.stride, p.norms, p.guard, n, d, k, row0 + r0,
+                      std::min(tile, rows - r0), nbr_idx, nbr_d2, kdist2, fb_rows, fb_count, s);
+        }
+        radix_knn(p.xs, p.stride, n, d, k, 0, 0, fb_rows, fb_count, nbr_idx, nbr_d2, kdist2, s);
+    } else {
+        for (int r0 = 0; r0 < rows; r0 += tile) {
+            radix_knn(p.xs, p.stride, n, d, k, row0 + r0, std::min(tile, rows - r0), nullptr,
+                      nullptr, nbr_idx, nbr_d2, kdist2, s);
+        }
+    }
+}
+
+void forward(const float* points, int n, int d, int k, bool normalize, int form, float* scores,
+             int* nbr_idx, float* nbr_d2, float* kdist2, float* lrd, hipStream_t s) {
+    if (form != CULOF_FORM_PLAIN) throw std::invalid_argument("unknown precision form");
+    int device = 0;
+    CULOF_CHECK(hipGetDevice(&device));
+    enable_pool_reuse(device);
+    Scratch scratch(s);
+    const size_t nn = static_cast<size_t>(n);
+    const size_t nk = nn * static_cast<size_t>(k);
+    if (!nbr_idx) nbr_idx = scratch.get<int>(nk);
+    if (!nbr_d2) nbr_d2 = scratch.get<float>(nk);
+    if (!kdist2) kdist2 = scratch.get<float>(nn);
+    if (!lrd) lrd = scratch.get<float>(nn);
+    const Prepared p = prepare(points, n, d, normalize, scratch, s);
+    knn_rows(p, n, d, k, 0, n, 0, nbr_idx, nbr_d2, kdist2, scratch, s);
+    compute_lrd(nbr_idx, nbr_d2, kdist2, n, k, lrd, s);
+    compute_scores(nbr_idx, lrd, n, k, scores, s);
+}
+
+std::vector<float> lof_host(const float* points, int n_points, int n_dims, int k, bool normalize) {
+    validate(points, n_points, n_dims, k);
+    require_device();
+    Stream stream;
+    const hipStream_t s = stream.get();
+    const size_t n = static_cast<size_t>(n_points);
+    std::vector<float> out(n);
+    {
+        Scratch scratch(s);
+        float* d_points = scratch.get<float>(n * n_dims);
+        float* d_scores = scratch.get<float>(n);
+        CULOF_CHECK(hipMemcpyAsync(d_points, points, n * n_dims * sizeof(float),
+                                   hipMemcpyHostToDevice, s));
+        forward(d_points, n_points, n_dims, k, normalize, CULOF_FORM_PLAIN, d_scores, nullptr,
+                nullptr, nullptr, nullptr, s);
+        CULOF_CHECK(hipMemcpyAsync(out.data(), d_scores, n * sizeof(float), hipMemcpyDeviceToHost,
+                                   s));
+    }
+    CULOF_CHECK(hipStreamSynchronize(s));
+    return out;
+}
+
+std::string describe_device() {
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return "no HIP device";
+    hipDeviceProp_t prop{};
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return "unknown device";
+    std::ostringstream os;
+    os << prop.name << " (" << prop.gcnArchName << ", " << prop.multiProcessorCount << " CUs, "
+       << (prop.totalGlobalMem >> 20) << " MiB)";
+    return os.str();
+}
+
+// ---------------------------------------------------------------------------
+// C ABI plumbing: exceptions -> status codes + per-thread message.
+// ---------------------------------------------------------------------------
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        g_last_error.clear();
+        f();
+        return CULOF_OK;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return CULOF_INVALID_ARGUMENT;
+    } catch (const CudaError& e) {
+        g_last_error = e.what();
+        return device_present() ? CULOF_DEVICE_ERROR : CULOF_NO_DEVICE;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return CULOF_DEVICE_ERROR;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return CULOF_DEVICE_ERROR;
+    }
+}
+
+hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+}  // namespace
+}  // namespace detail
+
+// ---------------------------------------------------------------------------
+// C++ API (include/culof.h)
+// ---------------------------------------------------------------------------
+
+bool cuda_available() { return detail::device_present(); }
+
+std::string device_info() { return detail::describe_device(); }
+
+std::vector<float> lof(const float* points, int n_points, int n_dims, int k, bool normalize) {
+    return detail::lof_host(points, n_points, n_dims, k, normalize);
+}
+
+}  // namespace culof
+
+// ---------------------------------------------------------------------------
+// C ABI (include/culof_c.h)
+// ---------------------------------------------------------------------------
+
+using namespace culof;
+using namespace culof::detail;
+
+extern "C" {
+
+const char* culof_last_error(void) { return g_last_error.c_str(); }
+
+int culof_device_available(void) { return device_present() ? 1 : 0; }
+
+int culof_device_info(char* buf, size_t capacity) {
+    return guarded([&] {
+        if (!buf || capacity == 0) throw std::invalid_argument("buffer must not be empty");
+        const std::string s = describe_device();
+        const size_t len = std::min(capacity - 1, s.size());
+        std::memcpy(buf, s.data(), len);
+        buf[len] = '\0';
+    });
+}
+
+int culof_lof(const float* points, int n_points, int n_dims, int k, int normalize,
+              float* scores) {
+    return guarded([&] {
+        if (!scores) throw std::invalid_argument("scores must not be null");
+        const std::vector<float> out = lof_host(points, n_points, n_dims, k, normalize != 0);
+        std::memcpy(scores, out.data(), out.size() * sizeof(float));
+    });
+}
+
+int culof_lof_device(const float* points, int n_points, int n_dims, int k, int normalize,
+                     int form, float* scores, int* nbr_idx, float* nbr_d2, float* kdist2,
+                     float* lrd, void* stream) {
+    return guarded([&] {
+        validate(points, n_points, n_dims, k);
+        if (!scores) throw std::invalid_argument("scores must not be null");
+        require_device();
+        forward(points, n_points, n_dims, k, normalize != 0, form, scores, nbr_idx, nbr_d2,
+                kdist2, lrd, as_stream(stream));
+    });
+}
+
+int culof_stage_center(const float* points, int n_points, int n_dims, int normalize,
+                       float* centred, float* norms, void* stream) {
+    return guarded([&] {
+        validate(points, n_points, n_dims, 1);
+        if (!centred || !norms) throw std::invalid_argument("outputs must not be null");
+        require_device();
+        const hipStream_t s = as_stream(stream);
+        Scratch scratch(s);
+        double* work = scratch.get<double>(static_cast<size_t>(258) * n_dims);
+        center_and_norms(points, n_points, n_dims, normalize != 0, centred, n_dims, norms, work,
+                         nullptr, nullptr, s);
+    });
+}
+
+int culof_stage_knn(const float* points, int n_points, int n_dims, int k, int normalize,
+                    int row0, int rows, int method, int* nbr_idx, float* nbr_d2, float* kdist2,
+                    void* stream) {
+    return guarded([&] {
+        validate(points, n_points, n_dims, k);
+        if (!nbr_idx || !nbr_d2 || !kdist2)
+            throw std::invalid_argument("outputs must not be null");
+        if (row0 < 0 || rows < 0 || row0 + rows > n_points)
+            throw std::invalid_argument("row range outside [0, n_points)");
+        if (method != 0 && method != 1) throw std::invalid_argument("method must be 0 or 1");
+        if (method == 0 && !sweep_supported(n_dims, k))
+            throw std::invalid_argument("no sweep instantiation for this (n_dims, k)");
+        require_device();
+        const hipStream_t s = as_stream(stream);
+        Scratch scratch(s);
+        const Prepared p = prepare(points, n_points, n_dims, normalize != 0, scratch, s);
+        knn_rows(p, n_points, n_dims, k, row0, rows, method, nbr_idx, nbr_d2, kdist2, scratch, s);
+    });
+}
+
+int culof_stage_select_rows(const float* keys, int rows, int n_cols, int k, int* nbr_idx,
+                            float* nbr_d2, float* kdist2, void* stream) {
+    return guarded([&] {
+        if (!keys || !nbr_idx || !nbr_d2 || !kdist2)
+            throw std::invalid_argument("inputs and outputs must not be null");
+        if (rows < 1 || n_cols < 1) throw std::invalid_argument("rows and n_cols must be >= 1");
+        if (k < 1 || k > n_cols) throw std::invalid_argument("k must satisfy 1 <= k <= n_cols");
+        require_device();
+        radix_select_rows(keys, rows, n_cols, k, nbr_idx, nbr_d2, kdist2, as_stream(stream));
+    });
+}
+
+int culof_stage_lrd(const int* nbr_idx, const float* nbr_d2, const float* kdist2, int n_points,
+                    int k, float* lrd, void* stream) {
+    return guarded([&] {
+        if (!nbr_idx || !nbr_d2 || !kdist2 || !lrd)
+            throw std::invalid_argument("inputs and outputs must not be null");
+        if (n_points < 1 || k < 1) throw std::invalid_argument("n_points and k must be >= 1");
+        require_device();
+        compute_lrd(nbr_idx, nbr_d2, kdist2, n_points, k, lrd, as_stream(stream));
+    });
+}
+
+int culof_stage_scores(const int* nbr_idx, const float* lrd, int n_points, int k, float* scores,
+                       void* stream) {
+    return guarded([&] {
+        if (!nbr_idx || !lrd || !scores)
+            throw std::invalid_argument("inputs and outputs must not be null");
+        if (n_points < 1 || k < 1) throw std::invalid_argument("n_points and k must be >= 1");
+        require_device();
+        compute_scores(nbr_idx, lrd, n_points, k, scores, as_stream(stream));
+    });
+}
+
+int culof_choose_tile_rows(int n_points, size_t budget_bytes) {
+    return detail::choose_tile_rows(n_points, budget_bytes);
+}
+
+int culof_profile_enable(int on) {
+    g_prof_on.store(on != 0);
+    return CULOF_OK;
+}
+
+int culof_profile_reset(void) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lock(g_prof_mu);
+        for (ProfRecord& r : g_prof) {
+            (void)hipEventSynchronize(r.stop);
+            (void)hipEventDestroy(r.start);
+            (void)hipEventDestroy(r.stop);
+        }
+        g_prof.clear();
+    });
+}
+
+int culof_profile_read(const char* name, double* total_ms, long* launches) {
+    return guarded([&] {
+        if (!name || !total_ms || !launches) throw std::invalid_argument("null argument");
+        std::lock_guard<std::mutex> lock(g_prof_mu);
+        double sum = 0.0;
+        long count = 0;
+        for (ProfRecord& r : g_prof) {
+            if (r.name.find(name) == std::string::npos) continue;
+            CULOF_CHECK(hipEventSynchronize(r.stop));
+            float ms = 0.0f;
+            CULOF_CHECK(hipEventElapsedTime(&ms, r.start, r.stop));
+            sum += ms;
+            ++count;
+        }
+        *total_ms = sum;
+        *launches = count;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,92 @@
+// Internal stage interfaces of libculof_amd (HIP, gfx950).
+//
+// Counterpart of the reference's include/culof_internal.cuh:31-64, taking a
+// hipStream_t. Every launcher checks its own launch and throws CudaError on a
+// HIP failure; the C ABI layer (api.cpp) turns exceptions into status codes.
+
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "culof.h"
+
+namespace culof {
+namespace detail {
+
+void check_hip(hipError_t err, const char* file, int line, const char* expr);
+
+#define CULOF_CHECK(expr) ::culof::detail::check_hip((expr), __FILE__, __LINE__, #expr)
+
+// Bit pattern of +inf: the key of a point against itself and of a NaN expansion.
+constexpr uint32_t kInfBits = 0x7F800000u;
+
+// Feature count padded for the fused sweep: the centred rows are stored with
+// this stride, zero-filled, so a row loads as whole float4s.
+int padded_dims(int n_dims);
+
+// ---- kernel timing hooks (see culof_profile_* in culof_c.h) ----
+struct ProfileScope {
+    ProfileScope(const char* name, hipStream_t stream);
+    ~ProfileScope();
+    ProfileScope(const ProfileScope&) = delete;
+    ProfileScope& operator=(const ProfileScope&) = delete;
+
+private:
+    int slot_ = -1;
+    hipStream_t stream_ = nullptr;
+};
+
+// ---- prep.hip: FP64 centring and norms (reference src/cuda/preprocess.cu) ----
+// Writes `centred` with row stride `stride` (>= n_dims, padding zeroed) and
+// FP64-accumulated squared norms. `work` must hold (256 + 2) * n_dims doubles.
+// Optional: `copy` receives the input rows at the same stride (zero padding);
+// *guard (zeroed by the caller) is set when some norm exceeds FLT_MAX / 16.
+void center_and_norms(const float* points, int n_points, int n_dims, bool scale, float* centred,
+                      int stride, float* norms, double* work, float* copy, int* guard,
+                      hipStream_t stream);
+
+// Selection keys are difference-form FP32 squared distances
+// e(i, j) = sum_t (a_t - b_t)^2 over the key rows (the input rows, or the
+// z-scored rows when normalising), an FMA chain in ascending t; +inf on the
+// diagonal and for NaN. Both kNN paths below select exactly by (e, index).
+
+// ---- sweep.hip: fused Gram screen + exact top-k per query row ----
+// Returns false when (n_dims, k) has no sweep instantiation; the caller then
+// uses the radix path for every row.
+bool sweep_supported(int n_dims, int k);
+// Queries [row0, row0 + rows). `keyrows` and `centred` share the padded
+// stride; `norms` are of the centred rows; *guard set means every row goes to
+// the fallback list. Rows whose k-th key is +inf are appended to
+// fallback_rows with fallback_count incremented; the radix path redoes them.
+void sweep_knn(const float* keyrows, const float* centred, int stride, const float* norms,
+               const int* guard, int n_points, int n_dims, int k, int row0, int rows,
+               int* nbr_idx, float* nbr_d2, float* kdist2, int* fallback_rows,
+               int* fallback_count, hipStream_t stream);
+
+// ---- radix.hip: per-row exact radix select (reference src/cuda/knn.cu) ----
+// Recompute mode: keys rebuilt from `keyrows` (row stride `stride`) every
+// pass. With `row_count` (device) null the query rows are [row0, row0 + rows);
+// otherwise they are row_list[0 .. *row_count) and the launch is a fixed-size
+// grid-stride loop, so no host read-back is needed.
+void radix_knn(const float* keyrows, int stride, int n_points, int n_dims, int k, int row0,
+               int rows, const int* row_list, const int* row_count, int* nbr_idx, float* nbr_d2,
+               float* kdist2, hipStream_t stream);
+// Stored mode: keys row-major (rows x n_cols); outputs with stride `rows`.
+void radix_select_rows(const float* keys, int rows, int n_cols, int k, int* nbr_idx,
+                       float* nbr_d2, float* kdist2, hipStream_t stream);
+
+// ---- density.hip: reference src/cuda/lof.cu ----
+void compute_lrd(const int* nbr_idx, const float* nbr_d2, const float* kdist2, int n_points,
+                 int k, float* lrd, hipStream_t stream);
+void compute_scores(const int* nbr_idx, const float* lrd, int n_points, int k, float* scores,
+                    hipStream_t stream);
+
+// Same arithmetic as the reference's choose_tile_rows (src/cuda/pipeline.cu:33-38).
+int choose_tile_rows(int n_points, size_t budget_bytes);
+
+}  // namespace detail
+}  // namespace culof

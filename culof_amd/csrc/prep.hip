@@ -1,0 +1,151 @@
+// FP64 centring (optionally z-scoring) and FP64-accumulated row norms.
+//
+// Semantics follow the reference's src/cuda/preprocess.cu:31-84: centring is
+// unconditional, scaling uses the population standard deviation with a 1e-12
+// guard, and the norms are taken over the centred float32 rows.
+//
+// The FP64 summation order is the reference's too, so the means come out bit
+// for bit the same: per feature, 256 partial sums where partial p holds rows
+// p, p + 256, p + 512, ... in ascending order, then a halving tree over the
+// partials. Here the 256 x d partials are independent threads (coalesced:
+// consecutive threads read consecutive features of a row) instead of one block
+// per feature reading a strided column, and the tree runs in a second tiny
+// kernel. The apply pass writes the padded row layout the sweep kernel reads.
+
+#include "internal.hpp"
+
+namespace culof {
+namespace detail {
+
+namespace {
+
+constexpr int kLanes = 256;  // partial sums per feature (the reference's block size)
+
+// Partial p of feature f. With `mean` set, sums squared deviations instead.
+__global__ void __launch_bounds__(256)
+column_partials_kernel(const float* __restrict__ x, int n, int d, const double* __restrict__ mean,
+                       double* __restrict__ part) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= kLanes * d) return;
+    const int f = t % d;
+    const int p = t / d;
+    const bool dev = mean != nullptr;
+    const double m = dev ? mean[f] : 0.0;
+    double s = 0.0;
+    int i = p;
+    // Eight independent loads in flight, added strictly in row order.
+    for (; i + 7 * kLanes < n; i += 8 * kLanes) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = x[static_cast<size_t>(i + u * kLanes) * d + f];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (dev) {
+                const double dv = static_cast<double>(v[u]) - m;
+                s = fma(dv, dv, s);
+            } else {
+                s += static_cast<double>(v[u]);
+            }
+        }
+    }
+    for (; i < n; i += kLanes) {
+        const float v = x[static_cast<size_t>(i) * d + f];
+        if (dev) {
+            const double dv = static_cast<double>(v) - m;
+            s = fma(dv, dv, s);
+        } else {
+            s += static_cast<double>(v);
+        }
+    }
+    part[static_cast<size_t>(p) * d + f] = s;
+}
+
+// One block per feature: halving tree over the 256 partials.
+// pass 0 writes mean = sum / n; pass 1 writes inv_std from the variance.
+__global__ void __launch_bounds__(256)
+column_finalize_kernel(const double* __restrict__ part, int n, int d, int pass,
+                       double* __restrict__ mean, double* __restrict__ inv_std) {
+    const int f = blockIdx.x;
+    __shared__ double s[kLanes];
+    s[threadIdx.x] = part[static_cast<size_t>(threadIdx.x) * d + f];
+    __syncthreads();
+    for (int off = kLanes / 2; off > 0; off >>= 1) {
+        if (static_cast<int>(threadIdx.x) < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (pass == 0) {
+            mean[f] = s[0] / n;
+            inv_std[f] = 1.0;
+        } else {
+            const double sd = sqrt(s[0] / n);
+            // A constant feature stays centred at zero rather than divided by noise.
+            inv_std[f] = (sd > 1e-12) ? 1.0 / sd : 1.0;
+        }
+    }
+}
+
+// One thread per row: centred float32 row (zero-padded to `stride`), its
+// norm, and optionally a zero-padded copy of the input row (the key rows when
+// not normalising). Raises *guard when a norm is too large for the Gram
+// screen to stay finite (above FLT_MAX / 16, or not finite).
+__global__ void __launch_bounds__(256)
+center_apply_kernel(const float* __restrict__ x, int n, int d, const double* __restrict__ mean,
+                    const double* __restrict__ inv_std, float* __restrict__ out, int stride,
+                    float* __restrict__ norms, float* __restrict__ copy, int* __restrict__ guard) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* row = x + static_cast<size_t>(i) * d;
+    float* dst = out + static_cast<size_t>(i) * stride;
+    float* cpy = copy ? copy + static_cast<size_t>(i) * stride : nullptr;
+    double s = 0.0;
+    for (int t = 0; t < d; ++t) {
+        const float xv = row[t];
+        const float v = static_cast<float>((static_cast<double>(xv) - mean[t]) * inv_std[t]);
+        dst[t] = v;
+        if (cpy) cpy[t] = xv;
+        s += static_cast<double>(v) * static_cast<double>(v);  // exact product
+    }
+    for (int t = d; t < stride; ++t) {
+        dst[t] = 0.0f;
+        if (cpy) cpy[t] = 0.0f;
+    }
+    const float nf = static_cast<float>(s);
+    norms[i] = nf;
+    if (guard && !(nf <= 2.1267647e37f)) atomicOr(guard, 1);  // FLT_MAX / 16, NaN
+}
+
+}  // namespace
+
+void center_and_norms(const float* points, int n_points, int n_dims, bool scale, float* centred,
+                      int stride, float* norms, double* work, float* copy, int* guard,
+                      hipStream_t stream) {
+    double* part = work;                                   // 256 x d
+    double* mean = work + static_cast<size_t>(kLanes) * n_dims;
+    double* inv_std = mean + n_dims;
+    const int threads = kLanes * n_dims;
+    const int grid = (threads + 255) / 256;
+    {
+        ProfileScope prof("center_partials", stream);
+        column_partials_kernel<<<grid, 256, 0, stream>>>(points, n_points, n_dims, nullptr, part);
+        CULOF_CHECK(hipGetLastError());
+    }
+    column_finalize_kernel<<<n_dims, kLanes, 0, stream>>>(part, n_points, n_dims, 0, mean, inv_std);
+    CULOF_CHECK(hipGetLastError());
+    if (scale) {
+        column_partials_kernel<<<grid, 256, 0, stream>>>(points, n_points, n_dims, mean, part);
+        CULOF_CHECK(hipGetLastError());
+        column_finalize_kernel<<<n_dims, kLanes, 0, stream>>>(part, n_points, n_dims, 1, mean,
+                                                              inv_std);
+        CULOF_CHECK(hipGetLastError());
+    }
+    {
+        ProfileScope prof("center_apply", stream);
+        center_apply_kernel<<<(n_points + 255) / 256, 256, 0, stream>>>(
+            points, n_points, n_dims, mean, inv_std, centred, stride, norms, copy, guard);
+        CULOF_CHECK(hipGetLastError());
+    }
+}
+
+}  // namespace detail
+}  // namespace culof

@@ -1,0 +1,133 @@
+/*
+ * culof C ABI: the FFI boundary of libculof_amd.so.
+ *
+ * Plain pointers, sizes and status codes; no C++ types, no torch types. Every
+ * entry point returns CULOF_OK (0) or a status code, and on failure leaves a
+ * message for culof_last_error() (per calling thread). Nothing here throws.
+ *
+ * Host-pointer entry points are synchronous. Device-pointer entry points
+ * (suffix _device and the stage functions) enqueue on `stream` (a hipStream_t,
+ * NULL for the default stream) and return without synchronising, except where
+ * a comment says otherwise.
+ *
+ * Layout conventions shared by every entry point:
+ *   points   row-major (n_points x n_dims) float32
+ *   nbr_idx  int32, transposed: neighbour slot t of point i at [t * n_points + i]
+ *   nbr_d2   float32, same layout, squared distance of that neighbour
+ *   kdist2   float32 (n_points), squared k-distance
+ * Slot order inside a neighbour list is the reference's: every neighbour
+ * strictly closer than the k-th key in ascending index, then the neighbours
+ * tied with the k-th key in ascending index (src/cuda/knn.cu:190-224). Slot
+ * k-1 therefore always holds the k-th neighbour in (distance, index) order.
+ */
+
+#ifndef CULOF_C_H
+#define CULOF_C_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum culof_status {
+    CULOF_OK = 0,
+    CULOF_INVALID_ARGUMENT = 1, /* std::invalid_argument in the C++ API */
+    CULOF_DEVICE_ERROR = 2,     /* CudaError in the C++ API */
+    CULOF_NO_DEVICE = 3         /* CudaError("no ... device available") */
+};
+
+/* Precision forms of the distance stage. */
+enum culof_form {
+    CULOF_FORM_PLAIN = 0 /* FP32 keys throughout */
+};
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* culof_last_error(void);
+
+/* 1 if a usable GPU is present, else 0.
+ * Replaces culof::cuda_available (reference include/culof.h:28,
+ * src/cuda/pipeline.cu:115-118). */
+int culof_device_available(void);
+
+/* Writes a NUL-terminated description of the current device into buf.
+ * Replaces culof::device_info (reference include/culof.h:31,
+ * src/cuda/pipeline.cu:120-129). */
+int culof_device_info(char* buf, size_t capacity);
+
+/* LOF scores of host data, written to host `scores` (n_points floats).
+ * Replaces culof::lof (reference include/culof.h:46-47,
+ * src/cuda/pipeline.cu:131-207); same argument checks and semantics. */
+int culof_lof(const float* points, int n_points, int n_dims, int k, int normalize,
+              float* scores);
+
+/* Device-resident forward. `points` and every output are device pointers.
+ * Optional outputs (nbr_idx, nbr_d2, kdist2, lrd) may be NULL. The reference
+ * has no device-pointer entry point; this is culof::lof without the two host
+ * copies (src/cuda/pipeline.cu:159-160,203-205), for callers whose data already
+ * lives in HBM (the torch binding). Never synchronises. */
+int culof_lof_device(const float* points, int n_points, int n_dims, int k, int normalize,
+                     int form, float* scores, int* nbr_idx, float* nbr_d2, float* kdist2,
+                     float* lrd, void* stream);
+
+/* ---- stage entry points (device pointers), for tests and integrators ----
+ * They replace the internal stage interface of the reference
+ * (include/culof_internal.cuh:31-64). */
+
+/* Centre (and with `normalize` z-score) each feature with FP64 accumulation
+ * and write the centred float32 rows (n_points x n_dims) plus the
+ * FP64-accumulated squared norms of those rows.
+ * Replaces detail::standardize + detail::squared_norms
+ * (include/culof_internal.cuh:31,34; src/cuda/preprocess.cu:31-98). */
+int culof_stage_center(const float* points, int n_points, int n_dims, int normalize,
+                       float* centred, float* norms, void* stream);
+
+/* Exact kNN of query rows [row0, row0 + rows) over all n_points candidates of
+ * `points` (centred, and z-scored with `normalize`, internally). Keys are
+ * difference-form FP32 squared distances sum_t (a_t - b_t)^2 of the input rows
+ * (of the z-scored rows with `normalize`), +inf on the diagonal; ties go to
+ * the lower index. method 0 = fused Gram-screened sweep (the production path),
+ * method 1 = per-row radix select recomputing keys every pass; both return the
+ * same bits. Outputs use stride n_points (slot t of point i at
+ * [t * n_points + i]). Replaces the SGEMM + detail::select_knn tile step
+ * (src/cuda/pipeline.cu:175-195, include/culof_internal.cuh:52). */
+int culof_stage_knn(const float* points, int n_points, int n_dims, int k, int normalize,
+                    int row0, int rows, int method, int* nbr_idx, float* nbr_d2,
+                    float* kdist2, void* stream);
+
+/* Exact radix select over stored key rows: keys is row-major (rows x n_cols)
+ * float32 (non-negative or +inf; NaN is read as +inf). Outputs use stride
+ * `rows` (slot t of row r at [t * rows + r]). This is the reference's selection
+ * rule applied to a materialised tile (src/cuda/knn.cu:119-232). */
+int culof_stage_select_rows(const float* keys, int rows, int n_cols, int k, int* nbr_idx,
+                            float* nbr_d2, float* kdist2, void* stream);
+
+/* lrd[i] = float(1 / (mean_t sqrt(double(max(kdist2[j], nbr_d2))) + 1e-10)).
+ * Replaces detail::compute_lrd (src/cuda/lof.cu:32-46,66-71). */
+int culof_stage_lrd(const int* nbr_idx, const float* nbr_d2, const float* kdist2, int n_points,
+                    int k, float* lrd, void* stream);
+
+/* scores[i] = float(sum_t double(lrd[j]) / double(lrd[i]) / k).
+ * Replaces detail::compute_scores (src/cuda/lof.cu:49-62,73-78). */
+int culof_stage_scores(const int* nbr_idx, const float* lrd, int n_points, int k, float* scores,
+                       void* stream);
+
+/* Rows per tile that keep n_points * 4 bytes per row within budget_bytes,
+ * clamped to [1, n_points]. Same arithmetic as detail::choose_tile_rows
+ * (src/cuda/pipeline.cu:33-38). */
+int culof_choose_tile_rows(int n_points, size_t budget_bytes);
+
+/* ---- kernel timing (no reference counterpart; used by the benchmark) ----
+ * When enabled, every launch of the library's kernels is bracketed by HIP
+ * events on the stream it runs on. culof_profile_read synchronises those
+ * events and returns the summed duration and launch count of the kernels
+ * whose name contains `name`. */
+int culof_profile_enable(int on);
+int culof_profile_reset(void);
+int culof_profile_read(const char* name, double* total_ms, long* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CULOF_C_H */

@@ -1,0 +1,174 @@
+"""Bit-exact parity of the HIP path with the CPU oracle, stage by stage.
+
+Every comparison here is exact (bit patterns, not tolerances): the oracle
+restates the same FP32/FP64 arithmetic the kernels perform, so any difference
+is a bug. Inputs are seeded; shapes cover every sweep instantiation
+(padded d in {4, 8, 16, 32, 64, 128}, k buckets 8/16/32/64) and the radix
+path (k > 64, d > 128), plus k = 1, 2 and n - 1.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import make_data
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def _assert_same_bits(got, want, what):
+    got_b, want_b = _bits(got), _bits(want)
+    bad = np.flatnonzero(got_b.ravel() != want_b.ravel())
+    assert bad.size == 0, f"{what}: {bad.size} mismatches, first at {bad[:5]}: " \
+        f"got {np.asarray(got).ravel()[bad[:5]]} want {np.asarray(want).ravel()[bad[:5]]}"
+
+
+SHAPES = [
+    # (n, d, k, n_outliers, seed)
+    (300, 1, 5, 3, 1),
+    (500, 3, 20, 8, 2),
+    (777, 8, 20, 9, 3),
+    (1000, 8, 1, 10, 4),
+    (640, 8, 2, 6, 5),
+    (900, 16, 33, 9, 6),
+    (700, 32, 50, 7, 7),
+    (600, 64, 64, 6, 8),
+    (400, 100, 10, 4, 9),
+    (520, 128, 32, 5, 10),
+    (300, 130, 7, 3, 11),    # d beyond the sweep: radix for every row
+    (600, 4, 65, 6, 12),     # k beyond the sweep
+    (128, 2, 127, 4, 13),    # k = n - 1
+    (2000, 8, 20, 20, 14),
+]
+
+
+def _shape_id(s):
+    return f"n{s[0]}-d{s[1]}-k{s[2]}"
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=_shape_id)
+def test_forward_bit_exact(shape):
+    from _gpu import forward
+
+    n, d, k, n_out, seed = shape
+    X = make_data(n, d, seed, n_out)
+    got = forward(X, k)
+    want = oracle.lof(X, k, full=True)
+    np.testing.assert_array_equal(got["idx"].T, want["idx"])
+    _assert_same_bits(got["d2"].T, want["d2"], "nbr_d2")
+    _assert_same_bits(got["kdist2"], want["kdist2"], "kdist2")
+    _assert_same_bits(got["lrd"], want["lrd"], "lrd")
+    _assert_same_bits(got["scores"], want["scores"], "scores")
+
+
+@pytest.mark.parametrize("normalize", [False, True])
+@pytest.mark.parametrize("shape", [(1000, 3, 10, 5, 21), (513, 7, 16, 5, 22), (300, 200, 9, 3, 23)],
+                         ids=_shape_id)
+def test_center_bit_exact(shape, normalize):
+    from _gpu import center
+
+    n, d, _, n_out, seed = shape
+    X = make_data(n, d, seed, n_out) * np.float32(3.0) + np.float32(7.0)
+    X[:, 0] = 4.0  # a constant feature
+    xc, norms = center(X, normalize)
+    want_xc, want_norms, _, _ = oracle.center(X, normalize)
+    _assert_same_bits(xc, want_xc, "centred rows")
+    _assert_same_bits(norms, want_norms, "norms")
+
+
+@pytest.mark.parametrize("normalize", [False, True])
+@pytest.mark.parametrize("shape", [s for s in SHAPES if s[1] <= 128 and s[2] <= 64], ids=_shape_id)
+def test_sweep_matches_radix_and_oracle(shape, normalize):
+    from _gpu import knn
+
+    n, d, k, n_out, seed = shape
+    X = make_data(n, d, seed, n_out)
+    sw = knn(X, k, normalize=normalize, method=0)
+    rx = knn(X, k, normalize=normalize, method=1)
+    oi, od, ok = oracle.knn_rows(oracle.key_rows_of(X, normalize), k)
+    for got, label in ((sw, "sweep"), (rx, "radix")):
+        np.testing.assert_array_equal(got[0].T, oi, err_msg=f"{label} nbr_idx")
+        _assert_same_bits(got[1].T, od, f"{label} nbr_d2")
+        _assert_same_bits(got[2], ok, f"{label} kdist2")
+
+
+def test_partial_row_ranges():
+    """A launch over a sub-range of query rows computes exactly those rows."""
+    from _gpu import knn
+
+    X = make_data(1500, 8, 31, 15)
+    oi, od, ok = oracle.knn_rows(X, 20)
+    for row0, rows in ((0, 1), (1, 63), (64, 65), (700, 800), (1499, 1)):
+        gi, gd, gk = knn(X, 20, row0=row0, rows=rows)
+        np.testing.assert_array_equal(gi.T, oi[row0:row0 + rows])
+        _assert_same_bits(gk, ok[row0:row0 + rows], "kdist2")
+
+
+def _random_key_rows(rng, rows, cols):
+    """Non-negative keys with heavy ties: zeros, +inf, a few repeated values."""
+    pool = np.concatenate([
+        np.zeros(3, np.float32), np.full(2, np.inf, np.float32),
+        rng.uniform(0, 4, size=12).astype(np.float32),
+    ])
+    keys = rng.uniform(0, 100, size=(rows, cols)).astype(np.float32)
+    mask = rng.random((rows, cols)) < 0.4
+    keys[mask] = rng.choice(pool, size=int(mask.sum()))
+    # denormals and huge finite values exercise every radix digit
+    keys[:, 0] = np.float32(1e-40)
+    keys[:, 1] = np.float32(3e38)
+    return keys
+
+
+@pytest.mark.parametrize("cols,k", [(1, 1), (7, 3), (300, 1), (300, 37), (300, 299), (300, 300),
+                                    (5000, 64), (5000, 1000), (70000, 20)])
+def test_stored_select_bit_exact(cols, k):
+    from _gpu import select_rows
+
+    rng = np.random.default_rng(cols * 131 + k)
+    rows = 17
+    keys = _random_key_rows(rng, rows, cols) if cols >= 2 else np.zeros((rows, cols), np.float32)
+    gi, gd, gk = select_rows(keys, k)
+    for r in range(rows):
+        wi, wd, wk = oracle.select_row(keys[r], k)
+        np.testing.assert_array_equal(gi[:, r], wi, err_msg=f"row {r}")
+        _assert_same_bits(gd[:, r], wd, f"row {r} keys")
+        _assert_same_bits(gk[r], np.float32(wk), f"row {r} kth")
+
+
+def test_density_stages_bit_exact():
+    from _gpu import lrd, scores
+
+    X = make_data(900, 5, 41, 9)
+    want = oracle.lof(X, 15, full=True)
+    got_lrd = lrd(want["idx"].T.copy(), want["d2"].T.copy(), want["kdist2"])
+    _assert_same_bits(got_lrd, want["lrd"], "lrd")
+    got_scores = scores(want["idx"].T.copy(), want["lrd"])
+    _assert_same_bits(got_scores, want["scores"], "scores")
+
+
+def test_tiling_is_bit_exact(monkeypatch):
+    """CULOF_TILE_ROWS splits the sweep into launches; every row is independent."""
+    from _gpu import forward
+
+    X = make_data(700, 5, 13, 9)
+    whole = forward(X, 25)["scores"]
+    for rows in ("1", "7", "64", "699"):
+        monkeypatch.setenv("CULOF_TILE_ROWS", rows)
+        _assert_same_bits(forward(X, 25)["scores"], whole, f"tile_rows={rows}")
+
+
+def test_repeated_calls_bit_identical():
+    from _gpu import forward
+
+    X = make_data(512, 6, 17, 7)
+    first = forward(X, 20)
+    for _ in range(3):
+        again = forward(X, 20)
+        for key in first:
+            np.testing.assert_array_equal(first[key], again[key])

@@ -1,0 +1,219 @@
+"""Pins the CPU oracle before anything is compared against it.
+
+Each check is chosen so that a plausible mistake in the oracle fails it:
+* scikit-learn 1.7.2 FP64 golden vectors (tests/golden/) and the
+  scikit-learn docstring known-answer test;
+* closed forms: points equally spaced on a circle have LOF exactly 1 by
+  symmetry, and so do the two points of n = 2;
+* the selection rule against an independent brute-force lexicographic sort;
+* centring against exact sums, invariants of the z-score, and norms;
+* the Gram-screen error bound the fused sweep relies on, on adversarial data;
+* planted mutations (the oracle's ``mutation`` argument: off-by-one
+  k-distance, missing max() in the reach distance, inverted density ratio)
+  that every score check must catch.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN_CASES, load_golden, make_data, rel_err
+
+# scikit-learn's LocalOutlierFactor docstring example (sklearn/neighbors/_lof.py:175-182).
+KAT_X = np.array([[-1.1], [0.2], [101.1], [0.3]], np.float32)
+KAT_LOF = np.array([0.98214286, 1.03703704, 73.36970899, 0.98214286])
+
+
+def test_kat_docstring_fp64():
+    got = oracle.lof_fp64(KAT_X, 2)
+    np.testing.assert_allclose(got, KAT_LOF, rtol=1e-6)
+
+
+def test_kat_docstring_fp32():
+    got = oracle.lof(KAT_X, 2)
+    np.testing.assert_allclose(got, KAT_LOF, rtol=1e-6)
+
+
+@pytest.mark.parametrize("m,k", [(3, 1), (3, 4), (3, 7), (4, 5), (4, 15)])
+def test_hypercube_is_exactly_one(m, k):
+    """Vertices of {0, 3}^m: every point sees the same distances, so every
+    k-distance, every density and every LOF is equal -- exactly 1 -- whatever
+    the tie-breaking picks."""
+    X = (np.array(np.meshgrid(*[[0, 1]] * m)).reshape(m, -1).T * 3).astype(np.float32)
+    assert np.all(oracle.lof_fp64(X, k) == 1.0)
+    assert np.all(oracle.lof(X, k) == 1.0)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 5, 8])
+def test_circle_is_one(k):
+    """Equally spaced points on a circle: LOF 1 up to the float32 rounding of
+    the positions themselves."""
+    n = 64
+    ang = 2 * np.pi * np.arange(n) / n
+    X = np.stack([np.cos(ang), np.sin(ang)], 1).astype(np.float32) * 10
+    np.testing.assert_allclose(oracle.lof_fp64(X, k), 1.0, rtol=1e-6)
+    np.testing.assert_allclose(oracle.lof(X, k), 1.0, rtol=2e-6)
+
+
+def test_two_points():
+    X = np.array([[0.0, 0.0], [3.0, 4.0]], np.float32)
+    assert np.all(oracle.lof_fp64(X, 1) == 1.0)
+    assert np.all(oracle.lof(X, 1) == 1.0)
+    full = oracle.lof(X, 1, full=True)
+    assert full["kdist2"].tolist() == [25.0, 25.0]
+    assert full["idx"].ravel().tolist() == [1, 0]
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_fp64_oracle_matches_sklearn(case):
+    g = load_golden(case)
+    k = int(g["k"])
+    got = oracle.lof_fp64(g["X"], k)
+    assert rel_err(got, g["lof"]).max() < 1e-9
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_fp32_oracle_within_north_star_tolerance(case):
+    """Median 1e-6, p99 5e-6 against scikit-learn FP64, identical top-1% sets."""
+    g = load_golden(case)
+    k = int(g["k"])
+    got = oracle.lof(g["X"], k)
+    r = rel_err(got, g["lof"])
+    assert np.median(r) <= 1e-6 and np.percentile(r, 99) <= 5e-6, (np.median(r), np.percentile(r, 99))
+    top = max(1, len(r) // 100)
+    assert set(np.argsort(-got, kind="stable")[:top]) == set(np.argsort(-g["lof"], kind="stable")[:top])
+
+
+@pytest.mark.parametrize("mutation", [1, 2, 3])
+def test_planted_mutations_are_caught(mutation):
+    """Each planted mistake moves scores far outside every tolerance above."""
+    caught = []
+    for case in ("clusters_n2000_d8_k20", "gauss_n400_d3_k20", "kat_docstring"):
+        g = load_golden(case)
+        k = int(g["k"])
+        r32 = rel_err(oracle.lof(g["X"], k, mutation=mutation), g["lof"])
+        r64 = rel_err(oracle.lof_fp64(g["X"], k, mutation=mutation), g["lof"])
+        caught.append(r32.max() > 1e-3 and r64.max() > 1e-3)
+    assert any(caught)
+    # the clustered case alone catches every mutation
+    assert caught[0]
+
+
+def _brute_select(keys, k):
+    """Independent statement of the selection rule: sort by (key, index)."""
+    order = np.lexsort((np.arange(len(keys)), keys.view(np.uint32)))
+    chosen = order[:k]
+    kth = keys.view(np.uint32)[chosen[-1]]
+    lt = np.sort(chosen[keys.view(np.uint32)[chosen] < kth])
+    eq = np.sort(chosen[keys.view(np.uint32)[chosen] == kth])
+    return np.concatenate([lt, eq]), keys[chosen[-1]]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_select_row_matches_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 400))
+    keys = rng.uniform(0, 5, n).astype(np.float32)
+    # heavy ties, zeros and +inf
+    m = rng.random(n) < 0.5
+    keys[m] = rng.choice(np.array([0.0, 1.0, 2.5, np.inf], np.float32), size=int(m.sum()))
+    for k in sorted({1, 2, n // 2 or 1, n - 1 or 1, n}):
+        idx, d2, kth = oracle.select_row(keys, k)
+        want_idx, want_kth = _brute_select(keys, k)
+        np.testing.assert_array_equal(idx, want_idx)
+        np.testing.assert_array_equal(d2, keys[want_idx])
+        assert np.float32(kth).view(np.uint32) == np.float32(want_kth).view(np.uint32)
+
+
+def test_centring_exact_on_integer_data():
+    rng = np.random.default_rng(0)
+    X = rng.integers(-50, 50, size=(1000, 4)).astype(np.float32)
+    xc, norms, mean, inv = oracle.center(X)
+    np.testing.assert_array_equal(mean, X.astype(np.float64).sum(0) / 1000)  # sums are exact
+    np.testing.assert_array_equal(inv, 1.0)
+    np.testing.assert_array_equal(xc, (X.astype(np.float64) - mean).astype(np.float32))
+
+
+def test_centring_statistics():
+    X = make_data(3000, 5, 3, 30) * np.float32(4.0) + np.float32(1000.0)
+    X[:, 2] = 7.0
+    xc, norms, mean, inv = oracle.center(X, normalize=False)
+    exact = np.array([math.fsum(X[:, f].astype(np.float64)) / len(X) for f in range(5)])
+    np.testing.assert_allclose(mean, exact, rtol=1e-15)
+    assert np.all(np.abs(xc.astype(np.float64).mean(0)) < 1e-4)
+    z, _, _, inv_z = oracle.center(X, normalize=True)
+    sd = np.delete(X.astype(np.float64).std(0), 2)
+    np.testing.assert_allclose(np.delete(z.std(0), 2), 1.0, rtol=1e-5)
+    np.testing.assert_allclose(np.delete(inv_z, 2), 1 / sd, rtol=1e-12)
+    assert inv_z[2] == 1.0 and np.all(z[:, 2] == 0.0)  # constant feature stays centred at zero
+    # norms: FP64 sum of squares of the centred float rows, rounded once
+    want = (xc.astype(np.float64) ** 2).sum(1).astype(np.float32)
+    assert np.max(np.abs(norms - want) / want) <= 6e-8
+
+
+def test_keys_are_squared_distances():
+    X = make_data(400, 6, 5, 4)
+    for i in (0, 17, 399):
+        keys = oracle.key_row(X, i)
+        exact = ((X.astype(np.float64) - X[i].astype(np.float64)) ** 2).sum(1)
+        assert keys[i] == np.inf
+        mask = np.arange(len(X)) != i
+        assert np.max(np.abs(keys[mask] - exact[mask]) / exact[mask]) <= 8 * 2.0 ** -24
+    # symmetric bit for bit
+    assert oracle.key_row(X, 3)[9] == oracle.key_row(X, 9)[3]
+
+
+def _adversarial_sets():
+    rng = np.random.default_rng(11)
+    far = make_data(300, 8, 1, 3) * np.float32(0.01) + np.float32(40.0)  # tiny spread, large offset
+    clus = np.concatenate([rng.normal(c, 0.05, size=(60, 16)) for c in rng.uniform(-30, 30, (5, 16))]).astype(np.float32)
+    dups = np.repeat(make_data(40, 4, 2), 5, axis=0)
+    wide = make_data(200, 128, 3, 2)
+    return {"far": far, "clusters": clus, "duplicates": dups, "wide": wide}
+
+
+@pytest.mark.parametrize("name", ["far", "clusters", "duplicates", "wide"])
+def test_gram_screen_bound_holds(name):
+    """|gram - e| <= c (|x_i|^2 + |x_j|^2 + e), c = (d + 16) 2^-24: the bound
+    the fused sweep screens with (centred rows, FP64-accumulated norms)."""
+    X = _adversarial_sets()[name]
+    n, d = X.shape
+    xc, norms, _, _ = oracle.center(X)
+    c = (d + 16) * 2.0 ** -24
+    worst = 0.0
+    for i in range(0, n, max(1, n // 40)):
+        e = oracle.key_row(X, i).astype(np.float64)
+        raw = oracle.gram_key_row(xc, norms, i).astype(np.float64)
+        mask = np.arange(n) != i
+        # gram keys clamp at 0, which only moves them toward e
+        slack = c * (norms[i] + norms[mask].astype(np.float64) + e[mask])
+        ratio = np.abs(raw[mask] - e[mask]) / slack
+        worst = max(worst, float(ratio.max()))
+    assert worst <= 1.0, worst
+
+
+def test_translation_and_permutation_invariance():
+    # coordinates on a 1/64 grid, so the shifted copy is exact in float32
+    X = np.round(make_data(600, 3, 8, 6) * 64) / 64
+    base = oracle.lof_fp64(X, 15)
+    shifted = (X + 512.0).astype(np.float32)
+    np.testing.assert_allclose(oracle.lof_fp64(shifted, 15), base, rtol=1e-12)
+    # far from the origin the FP32 keys keep their accuracy (no Gram cancellation)
+    far = (make_data(1500, 3, 7, 15) + np.float32(5000.0)).astype(np.float32)
+    assert rel_err(oracle.lof(far, 20), oracle.lof_fp64(far, 20)).max() < 2e-4
+    # permutation equivariance, away from ties (unquantised data)
+    Y = make_data(600, 3, 8, 6)
+    perm = np.random.default_rng(0).permutation(len(Y))
+    np.testing.assert_allclose(oracle.lof_fp64(Y[perm], 15), oracle.lof_fp64(Y, 15)[perm], rtol=1e-12)
+    np.testing.assert_allclose(oracle.lof(Y[perm], 15), oracle.lof(Y, 15)[perm], rtol=1e-6)
+
+
+def test_rejects_bad_arguments():
+    X = make_data(50, 2, 1)
+    for k in (0, 50):
+        with pytest.raises(ValueError):
+            oracle.lof(X, k)
