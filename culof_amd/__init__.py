@@ -22,6 +22,15 @@ from typing import TYPE_CHECKING
 
 import numpy as np
 
+# One process holds one HIP runtime. PyTorch-ROCm ships its own copy of it under
+# the same SONAME as /opt/rocm's; importing torch first makes libculof_amd bind
+# to that copy rather than load a second runtime (which would then own the GPU
+# alone), so torch tensors and this library share devices, memory and streams.
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover - numpy-only installs
+    pass
+
 try:
     from . import _culof
 except ImportError as exc:  # pragma: no cover - exercised only on unbuilt trees
