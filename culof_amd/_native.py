@@ -21,6 +21,7 @@ CULOF_DEVICE_ERROR = 2
 CULOF_NO_DEVICE = 3
 
 FORM_PLAIN = 0
+FORM_BF16 = 1
 
 # Every symbol include/culof_c.h declares, with (restype, argtypes).
 _c_int, _c_long, _c_double = ctypes.c_int, ctypes.c_long, ctypes.c_double
@@ -32,7 +33,7 @@ SIGNATURES = {
     "culof_lof": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp]),
     "culof_lof_device": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "culof_stage_center": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
-    "culof_stage_knn": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp]),
+    "culof_stage_knn": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp]),
     "culof_stage_select_rows": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp]),
     "culof_stage_lrd": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _vp, _vp]),
     "culof_stage_scores": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _vp]),

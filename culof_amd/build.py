@@ -35,6 +35,9 @@ HIP_FLAGS = [
     "-fPIC",
     # Packed FP32 VALU ops stay off: plain float code is compiled to scalar ops.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # Keep MFMA accumulators in VGPRs: the screen reads every result on the
+    # VALU, and AGPR copies would cost two moves per element per tile.
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wall",
     "-Wno-unused-result",
     f"-I{INCLUDE}",

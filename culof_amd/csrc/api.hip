@@ -3,10 +3,11 @@
 //
 // The forward pass is the reference's culof::lof (src/cuda/pipeline.cu:131-207)
 // re-planned for one MI355X:
-//   centre + norms (prep.hip) -> fused sweep kNN for every query row
-//   (sweep.hip; no distance tile exists, so memory is O(n k)) -> exact radix
-//   redo of any row the sweep hands back (radix.hip; a fixed grid-stride
-//   launch, no host read-back) -> LRD -> score (density.hip).
+//   centre + norms (prep.hip) -> screened exact kNN sweep for every query row
+//   (mfma.hip on the matrix cores, or sweep.hip on the VALU; no distance tile
+//   exists, so memory is O(n k)) -> exact radix redo of any row a sweep hands
+//   back (radix.hip; a fixed grid-stride launch, no host read-back) -> LRD ->
+//   score (density.hip).
 // Scratch is stream-ordered (hipMallocAsync on the call's stream), so calls on
 // different streams or threads never share memory and nothing synchronises
 // the device mid-pipeline. CULOF_TILE_ROWS still splits the query rows into
@@ -178,15 +179,15 @@ int tile_rows_for(int n_points) {
 struct Prepared {
     int stride = 0;             // padded row length shared by both row sets
     const float* xs = nullptr;  // key rows: the input, or the z-scored rows
-    const float* xc = nullptr;  // centred (and scaled) rows for the Gram screen
+    const float* xc = nullptr;  // centred (and scaled) rows for the distance screen
     const float* norms = nullptr;
     const int* guard = nullptr;
 };
 
-Prepared prepare(const float* points, int n, int d, bool normalize, Scratch& scratch,
+Prepared prepare(const float* points, int n, int d, bool normalize, int form, Scratch& scratch,
                  hipStream_t s) {
     Prepared p;
-    p.stride = padded_dims(d);
+    p.stride = mfma_padded_dims(d, form);
     const size_t cells = static_cast<size_t>(n) * p.stride;
     float* xc = scratch.get<float>(cells);
     float* norms = scratch.get<float>(n);
@@ -203,31 +204,67 @@ Prepared prepare(const float* points, int n, int d, bool normalize, Scratch& scr
     return p;
 }
 
-// kNN of query rows [row0, row0 + rows); method 0 = fused sweep with exact
-// radix redo of handed-back rows, 1 = radix for every row.
-void knn_rows(const Prepared& p, int n, int d, int k, int row0, int rows, int method,
+// Which kNN path serves (d, k, form). Measured on MI355X: for narrow rows
+// (padded d <= 16) in the FP32 form the VALU-screened sweep is the faster one
+// (its 4 waves share one threshold per query); wider rows and the BF16 form go
+// to the matrix-core sweep; what neither instantiates goes to the radix path.
+// CULOF_KNN_PATH (mfma | valu | radix) pins a path, to benchmark the others.
+enum class Path { kMfma, kValu, kRadix };
+
+Path choose_path(int d, int k, int form, int stride) {
+    const bool mfma = mfma_supported(d, k, form);
+    const bool valu = form == kFormPlain && stride == padded_dims(d) && sweep_supported(d, k);
+    if (const char* env = std::getenv("CULOF_KNN_PATH")) {
+        const std::string want(env);
+        if (want == "radix") return Path::kRadix;
+        if (want == "valu" && valu) return Path::kValu;
+        if (want == "mfma" && mfma) return Path::kMfma;
+    }
+    if (valu && padded_dims(d) <= 16) return Path::kValu;
+    if (mfma) return Path::kMfma;
+    if (valu) return Path::kValu;
+    return Path::kRadix;
+}
+
+// kNN of query rows [row0, row0 + rows). method 0 = the production path for
+// `form` (see choose_path), 1 = radix for every row, 2 = VALU sweep.
+// Rows a sweep hands back get the exact radix redo.
+void knn_rows(const Prepared& p, int n, int d, int k, int form, int row0, int rows, int method,
               int* nbr_idx, float* nbr_d2, float* kdist2, Scratch& scratch, hipStream_t s) {
     const int tile = tile_rows_for(rows);
-    if (method == 0 && sweep_supported(d, k)) {
-        int* fb_count = scratch.get<int>(1);
-        int* fb_rows = scratch.get<int>(static_cast<size_t>(rows));
-        CULOF_CHECK(hipMemsetAsync(fb_count, 0, sizeof(int), s));
-        for (int r0 = 0; r0 < rows; r0 += tile) {
-            sweep_knn(p.xs, p.xc, p.stride, p.norms, p.guard, n, d, k, row0 + r0,
-                      std::min(tile, rows - r0), nbr_idx, nbr_d2, kdist2, fb_rows, fb_count, s);
-        }
-        radix_knn(p.xs, p.stride, n, d, k, 0, 0, fb_rows, fb_count, nbr_idx, nbr_d2, kdist2, s);
-    } else {
+    Path path = Path::kRadix;
+    if (method == 0) path = choose_path(d, k, form, p.stride);
+    if (method == 2) path = Path::kValu;
+    if (path == Path::kRadix) {
         for (int r0 = 0; r0 < rows; r0 += tile) {
             radix_knn(p.xs, p.stride, n, d, k, row0 + r0, std::min(tile, rows - r0), nullptr,
                       nullptr, nbr_idx, nbr_d2, kdist2, s);
         }
+        return;
     }
+    int* fb_count = scratch.get<int>(1);
+    int* fb_rows = scratch.get<int>(static_cast<size_t>(rows));
+    CULOF_CHECK(hipMemsetAsync(fb_count, 0, sizeof(int), s));
+    void* mfma_scratch = nullptr;
+    if (path == Path::kMfma) {
+        mfma_scratch = scratch.get<unsigned char>(mfma_scratch_bytes(n, p.stride, form, tile, k));
+    }
+    for (int r0 = 0; r0 < rows; r0 += tile) {
+        const int r = std::min(tile, rows - r0);
+        if (path == Path::kMfma) {
+            mfma_knn(p.xs, p.xc, p.stride, p.norms, p.guard, n, d, k, form, row0 + r0, r,
+                     mfma_scratch, nbr_idx, nbr_d2, kdist2, fb_rows, fb_count, s);
+        } else {
+            sweep_knn(p.xs, p.xc, p.stride, p.norms, p.guard, n, d, k, row0 + r0, r, nbr_idx,
+                      nbr_d2, kdist2, fb_rows, fb_count, s);
+        }
+    }
+    radix_knn(p.xs, p.stride, n, d, k, 0, 0, fb_rows, fb_count, nbr_idx, nbr_d2, kdist2, s);
 }
 
 void forward(const float* points, int n, int d, int k, bool normalize, int form, float* scores,
              int* nbr_idx, float* nbr_d2, float* kdist2, float* lrd, hipStream_t s) {
-    if (form != CULOF_FORM_PLAIN) throw std::invalid_argument("unknown precision form");
+    if (form != kFormPlain && form != kFormBF16) throw std::invalid_argument("unknown precision form");
     int device = 0;
     CULOF_CHECK(hipGetDevice(&device));
     enable_pool_reuse(device);
@@ -238,8 +275,8 @@ void forward(const float* points, int n, int d, int k, bool normalize, int form,
     if (!nbr_d2) nbr_d2 = scratch.get<float>(nk);
     if (!kdist2) kdist2 = scratch.get<float>(nn);
     if (!lrd) lrd = scratch.get<float>(nn);
-    const Prepared p = prepare(points, n, d, normalize, scratch, s);
-    knn_rows(p, n, d, k, 0, n, 0, nbr_idx, nbr_d2, kdist2, scratch, s);
+    const Prepared p = prepare(points, n, d, normalize, form, scratch, s);
+    knn_rows(p, n, d, k, form, 0, n, 0, nbr_idx, nbr_d2, kdist2, scratch, s);
     compute_lrd(nbr_idx, nbr_d2, kdist2, n, k, lrd, s);
     compute_scores(nbr_idx, lrd, n, k, scores, s);
 }
@@ -382,22 +419,24 @@ int culof_stage_center(const float* points, int n_points, int n_dims, int normal
 }
 
 int culof_stage_knn(const float* points, int n_points, int n_dims, int k, int normalize,
-                    int row0, int rows, int method, int* nbr_idx, float* nbr_d2, float* kdist2,
-                    void* stream) {
+                    int form, int row0, int rows, int method, int* nbr_idx, float* nbr_d2,
+                    float* kdist2, void* stream) {
     return guarded([&] {
         validate(points, n_points, n_dims, k);
         if (!nbr_idx || !nbr_d2 || !kdist2)
             throw std::invalid_argument("outputs must not be null");
         if (row0 < 0 || rows < 0 || row0 + rows > n_points)
             throw std::invalid_argument("row range outside [0, n_points)");
-        if (method != 0 && method != 1) throw std::invalid_argument("method must be 0 or 1");
-        if (method == 0 && !sweep_supported(n_dims, k))
-            throw std::invalid_argument("no sweep instantiation for this (n_dims, k)");
+        if (form != kFormPlain && form != kFormBF16) throw std::invalid_argument("unknown precision form");
+        if (method < 0 || method > 2) throw std::invalid_argument("method must be 0, 1 or 2");
+        if (method == 2 && (form != kFormPlain || !sweep_supported(n_dims, k)))
+            throw std::invalid_argument("no VALU sweep instantiation for this (n_dims, k, form)");
         require_device();
         const hipStream_t s = as_stream(stream);
         Scratch scratch(s);
-        const Prepared p = prepare(points, n_points, n_dims, normalize != 0, scratch, s);
-        knn_rows(p, n_points, n_dims, k, row0, rows, method, nbr_idx, nbr_d2, kdist2, scratch, s);
+        const Prepared p = prepare(points, n_points, n_dims, normalize != 0, form, scratch, s);
+        knn_rows(p, n_points, n_dims, k, form, row0, rows, method, nbr_idx, nbr_d2, kdist2,
+                 scratch, s);
     });
 }
 

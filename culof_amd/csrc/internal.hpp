@@ -24,6 +24,10 @@ void check_hip(hipError_t err, const char* file, int line, const char* expr);
 // Bit pattern of +inf: the key of a point against itself and of a NaN expansion.
 constexpr uint32_t kInfBits = 0x7F800000u;
 
+// Precision forms of the distance screen (culof_form in culof_c.h).
+constexpr int kFormPlain = 0;  // FP32 screen
+constexpr int kFormBF16 = 1;   // BF16 MFMA screen
+
 // Feature count padded for the fused sweep: the centred rows are stored with
 // this stride, zero-filled, so a row loads as whole float4s.
 int padded_dims(int n_dims);
@@ -66,6 +70,19 @@ void sweep_knn(const float* keyrows, const float* centred, int stride, const flo
                const int* guard, int n_points, int n_dims, int k, int row0, int rows,
                int* nbr_idx, float* nbr_d2, float* kdist2, int* fallback_rows,
                int* fallback_count, hipStream_t stream);
+
+// ---- mfma.hip: matrix-core screened sweep (FP32 or BF16 screen) ----
+// Row stride the form's screen expects (BF16 needs multiples of 16).
+int mfma_padded_dims(int n_dims, int form);
+bool mfma_supported(int n_dims, int k, int form);
+int mfma_slices(int n_points, int rows);
+// Device scratch the launch needs (screen operand copy, adjusted norms,
+// per-slice partial lists).
+size_t mfma_scratch_bytes(int n_points, int dp, int form, int rows, int k);
+void mfma_knn(const float* keyrows, const float* centred, int stride, const float* norms,
+              const int* guard, int n_points, int n_dims, int k, int form, int row0, int rows,
+              void* scratch, int* nbr_idx, float* nbr_d2, float* kdist2, int* fallback_rows,
+              int* fallback_count, hipStream_t stream);
 
 // ---- radix.hip: per-row exact radix select (reference src/cuda/knn.cu) ----
 // Recompute mode: keys rebuilt from `keyrows` (row stride `stride`) every

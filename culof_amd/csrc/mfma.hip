@@ -1,0 +1,727 @@
+// MFMA-screened exact kNN sweep: the production selection for every d <= 128
+// and k <= 64, in two precision forms of the screen.
+//
+// Same result as sweep.hip and radix.hip -- the k smallest difference-form
+// FP32 keys e(i, j) = sum_t (a_t - b_t)^2 per query, ties toward the lower
+// index, slot order of the reference (src/cuda/knn.cu:190-224) -- computed with
+// the distance screen on the matrix cores:
+//
+//   * A wave owns 32 queries; lanes l and l + 32 both serve query l % 32.
+//     The 4 waves of a block hold 4 query tiles and stream the same candidate
+//     tiles, so a tile fetched by one wave is an L1 hit for the others.
+//   * Per 32 x 32 tile the screen is one chain of MFMAs, C = m + A B with
+//     A = candidate rows (32 x DP), B = -2 x query rows (DP x 32) and the
+//     accumulator pre-loaded with m_i = |x_i|^2 (1 - c). Form FP32 uses
+//     v_mfma_f32_32x32x2_f32 on the centred rows; form BF16 uses
+//     v_mfma_f32_32x32x16_bf16 on RNE-rounded copies (products exact,
+//     FP32 accumulation). The query's own |x_q|^2 is folded into the threshold,
+//     so the VALU does one min per pair and one compare per tile.
+//   * A pair passes the screen when acc <= T (1 + c) + c |x_q|^2 - |x_q|^2, with
+//     T the lane's current k-th exact key and c the form's relative error bound
+//     (derivation below). Passing candidates are queued (index only) in a
+//     per-lane LDS ring; when some lane's ring is nearly full the wave computes
+//     their exact keys in a batch and inserts them into the lane's sorted
+//     register list, exactly as the VALU sweep does.
+//   * Grid = (query tiles / 4) x S candidate slices. With S > 1 each slice
+//     writes its per-query top-k as (key, index) words and merge_slices_kernel
+//     folds them; with S = 1 the sweep writes the final lists directly.
+//
+// Screen bound. Let u = 2^-24, N the exact squared norms of the centred rows.
+// The accumulated value differs from m_i - 2 x_q.x_i by at most 2 g (N_q + N_i),
+// g = (DP + 8) u covering any FP32 accumulation order of the MFMA chain and the
+// product roundings; BF16 rounding of both operands (|d| <= 2^-8) adds
+// (2^-7 + 2^-16) (N_q + N_i). With the norm, centring and exact-key roundings
+// (about 3.2 u N and (d + 5) u T) this gives
+//   FP32: c = (2 DP + 32) u,   BF16: c = 2^-7 + 2^-14 + (2 DP + 64) u,
+// and the threshold arithmetic adds a 4 u (T + |x_q|^2) margin for its own
+// rounding. A true top-k candidate therefore always passes; the exact keys
+// decide, so every form selects the same neighbours.
+
+#include "internal.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+#include <type_traits>
+
+namespace culof {
+namespace detail {
+
+namespace {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr uint64_t kSentinel = ~0ull;
+constexpr int kTilesPerBlock = 4;  // query tiles (waves) per block
+constexpr int kRing = 32;          // queued screen passes per lane
+constexpr float kU = 5.9604645e-8f;  // 2^-24
+
+__device__ __forceinline__ uint64_t pack(uint32_t key, uint32_t idx) {
+    return (static_cast<uint64_t>(key) << 32) | idx;
+}
+
+__device__ __forceinline__ float key_as_float(uint64_t w) {
+    const uint32_t k = static_cast<uint32_t>(w >> 32);
+    return __uint_as_float(k > kInfBits ? kInfBits : k);
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sorted register list, descending, threshold at a[0] (see sweep.hip).
+template <int KMAX>
+__device__ __forceinline__ void list_init(uint64_t (&a)[KMAX], int k) {
+#pragma unroll
+    for (int t = 0; t < KMAX; ++t) a[t] = (t < k) ? kSentinel : 0ull;
+}
+
+template <int KMAX>
+__device__ __forceinline__ void list_insert(uint64_t (&a)[KMAX], uint64_t x) {
+    a[0] = x;
+#pragma unroll
+    for (int t = 0; t + 1 < KMAX; ++t) {
+        const uint64_t hi = a[t] < a[t + 1] ? a[t + 1] : a[t];
+        const uint64_t lo = a[t] < a[t + 1] ? a[t] : a[t + 1];
+        a[t] = hi;
+        a[t + 1] = lo;
+    }
+}
+
+// ---- screen operand formats ----
+
+// FP32: row i stored as [x_0, x_2, ..., x_{DP-2}, x_1, x_3, ..., x_{DP-1}], so
+// the lane serving row i, k-half h, reads DP/2 contiguous floats for its
+// DP/2 MFMAs (instruction m consumes k = 2m + h).
+struct ScreenF32 {
+    template <int DP>
+    struct Frag {
+        float v[DP / 2];
+    };
+    using Elem = float;
+    // This lane's half of a row: DP / 2 contiguous floats.
+    template <int DP>
+    __device__ static void load_a(const Elem* half_row, Frag<DP>& f) {
+        if constexpr (DP % 8 == 0) {
+            const float4* p = reinterpret_cast<const float4*>(half_row);
+#pragma unroll
+            for (int v = 0; v < DP / 8; ++v) {
+                const float4 t = p[v];
+                f.v[4 * v + 0] = t.x;
+                f.v[4 * v + 1] = t.y;
+                f.v[4 * v + 2] = t.z;
+                f.v[4 * v + 3] = t.w;
+            }
+        } else {  // DP == 4: two floats
+            const float2 t = reinterpret_cast<const float2*>(half_row)[0];
+            f.v[0] = t.x;
+            f.v[1] = t.y;
+        }
+    }
+    __device__ static constexpr int half_offset(int dp, int h) { return h * (dp / 2); }
+    // B fragment of the query: -2 x_q[2m + h] (natural-order centred row).
+    template <int DP>
+    __device__ static void load_b(const float* xc_row, int h, Frag<DP>& f) {
+#pragma unroll
+        for (int m = 0; m < DP / 2; ++m) f.v[m] = -2.0f * xc_row[2 * m + h];
+    }
+    template <int DP>
+    __device__ static floatx16 run(const Frag<DP>& a, const Frag<DP>& b, floatx16 acc) {
+#pragma unroll
+        for (int m = 0; m < DP / 2; ++m) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.v[m], b.v[m], acc, 0, 0, 0);
+        return acc;
+    }
+    __host__ __device__ static float slack(int dp) { return (2.0f * dp + 32.0f) * kU; }
+};
+
+// BF16: row i stored as RNE-rounded bf16 in natural order; instruction m
+// consumes k in [16m, 16m + 16), the lane of k-half h the 8 values at 16m + 8h.
+struct ScreenBF16 {
+    template <int DP>
+    struct Frag {
+        bf16x8 v[DP / 16];
+    };
+    using Elem = __bf16;
+    // From this lane's first element of a row (offset 8h): 8 bf16 per MFMA.
+    template <int DP>
+    __device__ static void load_a(const Elem* half_row, Frag<DP>& f) {
+        const bf16x8* p = reinterpret_cast<const bf16x8*>(half_row);
+#pragma unroll
+        for (int m = 0; m < DP / 16; ++m) f.v[m] = p[2 * m];
+    }
+    __device__ static constexpr int half_offset(int, int h) { return 8 * h; }
+    template <int DP>
+    __device__ static void load_b(const float* xc_row, int h, Frag<DP>& f) {
+#pragma unroll
+        for (int m = 0; m < DP / 16; ++m) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) f.v[m][e] = static_cast<__bf16>(-2.0f * xc_row[16 * m + 8 * h + e]);
+        }
+    }
+    template <int DP>
+    __device__ static floatx16 run(const Frag<DP>& a, const Frag<DP>& b, floatx16 acc) {
+#pragma unroll
+        for (int m = 0; m < DP / 16; ++m) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.v[m], b.v[m], acc, 0, 0, 0);
+        return acc;
+    }
+    __host__ __device__ static float slack(int dp) {
+        return 0.0078125f + 6.1035156e-5f + (2.0f * dp + 64.0f) * kU;
+    }
+};
+
+// LDS chunk geometry: candidate tiles per chunk (up to 8, about 16 KiB of
+// screen operand per buffer) and the row pitch in float4s. Rows of 64 bytes
+// or more get one float4 of padding so the 32 rows a wave reads in one
+// ds_read_b128 fall in different banks.
+template <class Op, int DP>
+struct ChunkShape {
+    static constexpr int kRowF4 = DP * static_cast<int>(sizeof(typename Op::Elem)) / 16 > 0
+                                      ? DP * static_cast<int>(sizeof(typename Op::Elem)) / 16
+                                      : 1;
+    static constexpr int kPitchF4 = kRowF4 >= 4 ? kRowF4 + 1 : kRowF4;
+    static constexpr int kTileBytes = 32 * kPitchF4 * 16;
+    static constexpr int kFit = 16384 / kTileBytes;
+    static constexpr int kTiles = kFit >= 8 ? 8 : kFit >= 4 ? 4 : kFit >= 2 ? 2 : 1;  // a power of two
+    // A operand plus the tile norms (4 bytes per row)
+    static constexpr size_t kBytes = static_cast<size_t>(kTiles) * (kTileBytes + 32 * 4);
+};
+
+// Exact key of a query row against key row xs_row: the same FMA chain as the
+// oracle and the other sweeps. Q is either a register array (indexed with
+// constants only) or a pointer to the query's key row in global memory.
+template <int DP, class Q>
+__device__ __forceinline__ float exact_key(const Q& q, const float* __restrict__ xs_row) {
+    const float4* c4 = reinterpret_cast<const float4*>(xs_row);
+    float e = 0.0f;
+#pragma unroll
+    for (int v = 0; v < DP / 4; ++v) {
+        const float4 t = c4[v];
+        float df = q[4 * v + 0] - t.x;
+        e = __builtin_fmaf(df, df, e);
+        df = q[4 * v + 1] - t.y;
+        e = __builtin_fmaf(df, df, e);
+        df = q[4 * v + 2] - t.z;
+        e = __builtin_fmaf(df, df, e);
+        df = q[4 * v + 3] - t.w;
+        e = __builtin_fmaf(df, df, e);
+    }
+    return e;
+}
+
+// Writes one query's list (ascending in `stage`, k entries at stride 64) in the
+// reference's slot order: keys below the k-th ascending by index, then the
+// tied ones already in ascending index. Returns false for a +inf k-th key.
+__device__ bool emit_list(const uint64_t* stage, int k, int n, int qi, int* __restrict__ nbr_idx,
+                          float* __restrict__ nbr_d2, float* __restrict__ kdist2) {
+    const uint32_t kth = static_cast<uint32_t>(stage[(k - 1) * 64] >> 32);
+    if (kth >= kInfBits) return false;
+    kdist2[qi] = __uint_as_float(kth);
+    int below = 0;
+    for (int t = 0; t < k; ++t) below += static_cast<uint32_t>(stage[t * 64] >> 32) < kth;
+    const size_t ns = static_cast<size_t>(n);
+    for (int e = 0; e < k; ++e) {
+        const uint64_t w = stage[e * 64];
+        const uint32_t idx = static_cast<uint32_t>(w);
+        int slot = e;
+        if (e < below) {
+            slot = 0;
+            for (int f = 0; f < below; ++f) slot += static_cast<uint32_t>(stage[f * 64]) < idx;
+        }
+        nbr_idx[static_cast<size_t>(slot) * ns + qi] = static_cast<int>(idx);
+        nbr_d2[static_cast<size_t>(slot) * ns + qi] = __uint_as_float(static_cast<uint32_t>(w >> 32));
+    }
+    return true;
+}
+
+struct MfmaArgs {
+    const void* xa;      // screen operand rows (form layout)
+    const float* xc;     // centred rows, natural order, stride DP
+    const float* xs;     // key rows, stride DP
+    const float* madj;   // |x_i|^2 (1 - c)
+    const float* norms;  // |x_i|^2
+    const int* guard;
+    int n;
+    int k;
+    float c;
+    int row0;
+    int rows;
+    int slices;
+    uint64_t* partial;   // [slices][rows][k] when slices > 1
+    int* nbr_idx;
+    float* nbr_d2;
+    float* kdist2;
+    int* fb_rows;
+    int* fb_count;
+};
+
+// Wide lists need the whole register file: let those instantiations run at
+// two waves per SIMD rather than spill the list to scratch.
+template <class Op, int DP, int KMAX>
+__global__ void __launch_bounds__(64 * kTilesPerBlock)
+__attribute__((amdgpu_waves_per_eu(1, KMAX >= 64 ? 2 : 8)))
+mfma_sweep_kernel(MfmaArgs p) {
+    using Frag = typename Op::template Frag<DP>;
+    constexpr bool kQRegs = DP <= 16;  // query key row in VGPRs
+
+    // LDS: rings + two candidate chunks during the sweep; afterwards one wave
+    // at a time stages its lists in the same bytes.
+    constexpr size_t kRingBytes = sizeof(uint32_t) * kTilesPerBlock * kRing * 64;
+    constexpr size_t kChunkBytes = ChunkShape<Op, DP>::kBytes;
+    constexpr size_t kSweepBytes = kRingBytes + 2 * kChunkBytes;
+    constexpr size_t kStageBytes = sizeof(uint64_t) * KMAX * 64;
+    __shared__ __attribute__((aligned(16))) unsigned char smem[kSweepBytes > kStageBytes ? kSweepBytes : kStageBytes];
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int j = lane & 31;   // query within the tile
+    const int h = lane >> 5;   // k-half / candidate-row half
+    const int local = (blockIdx.x * kTilesPerBlock + wave) * 32 + j;
+    const bool valid = local < p.rows;
+    const int qi = p.row0 + (valid ? local : 0);
+    const int n = p.n;
+
+    if (*p.guard) {
+        if (valid && h == 0 && blockIdx.y == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+        return;
+    }
+
+    // Query operands.
+    Frag bq;
+    Op::template load_b<DP>(p.xc + static_cast<size_t>(qi) * DP, h, bq);
+    float qs[kQRegs ? DP : 1];
+    if constexpr (kQRegs) {
+#pragma unroll
+        for (int t = 0; t < DP; ++t) qs[t] = p.xs[static_cast<size_t>(qi) * DP + t];
+    }
+    const float* q_global = p.xs + static_cast<size_t>(qi) * DP;
+    const float nq = p.norms[qi];
+    const float c = p.c;
+
+    uint64_t a[KMAX];
+    list_init<KMAX>(a, p.k);
+    // Each of the two lanes serving a query keeps k exact keys over its half of
+    // the candidates, so the smaller of the two k-th keys bounds the final one:
+    // both screen against it (lane l and l ^ 32 swap theirs).
+    auto threshold = [&]() {
+        float t = key_as_float(a[0]);
+        t = fminf(t, __shfl_xor(t, 32, 64));
+        if (!valid) return -__uint_as_float(kInfBits);
+        return (__builtin_fmaf(t, 1.0f + c, c * nq) - nq) + 4.0f * kU * (t + nq);
+    };
+    float Tn = threshold();
+    int cnt = 0;
+    uint32_t* ring = reinterpret_cast<uint32_t*>(smem) + wave * kRing * 64;
+
+    // Candidate slice of this block, streamed in chunks of CT tiles through a
+    // double-buffered LDS stage shared by the block's 4 waves: while the waves
+    // screen chunk c out of one buffer, every thread holds its share of chunk
+    // c + 1 in registers and writes it to the other buffer afterwards; one
+    // barrier per chunk. The operand arrays are padded to whole tiles (zero
+    // rows, NaN norms); tiles past the slice end are never screened.
+    using Elem = typename Op::Elem;
+    using Shape = ChunkShape<Op, DP>;
+    constexpr int CT = Shape::kTiles;
+    constexpr int kRowsPerChunk = 32 * CT;
+    constexpr int kRowF4 = Shape::kRowF4;     // float4s per global row (DP * elem >= 16 bytes)
+    constexpr int kPitchF4 = Shape::kPitchF4; // float4s per LDS row
+    constexpr int kTileF4 = 32 * kRowF4;      // float4s per A tile in global memory
+    constexpr int kChunkF4 = CT * kTileF4;
+    constexpr int kLdsTileF4 = 32 * kPitchF4;
+    constexpr int kLdsChunkF4 = CT * kLdsTileF4;
+    constexpr int kLoadsA = (kChunkF4 + 255) / 256;  // per thread
+    constexpr int kChunkM4 = kRowsPerChunk / 4;       // float4s of norms per chunk
+    static_assert(DP * sizeof(Elem) % 16 == 0, "screen rows must be whole float4s");
+    float4* s_a = reinterpret_cast<float4*>(smem + kRingBytes);
+    float4* s_m = s_a + 2 * kLdsChunkF4;
+
+    const int n_tiles = (n + 31) / 32;
+    const int t_begin = static_cast<int>((static_cast<long long>(n_tiles) * blockIdx.y) / p.slices);
+    const int t_end = static_cast<int>((static_cast<long long>(n_tiles) * (blockIdx.y + 1)) / p.slices);
+    const int n_chunks = (t_end - t_begin + CT - 1) / CT;
+    const float4* ga = static_cast<const float4*>(p.xa);
+    const float4* gm = reinterpret_cast<const float4*>(p.madj);
+    const float nan = __uint_as_float(0x7FC00000u);
+
+    float4 ra[kLoadsA];
+    float4 rm = make_float4(nan, nan, nan, nan);
+    auto fetch = [&](int t0) {
+#pragma unroll
+        for (int u = 0; u < kLoadsA; ++u) {
+            const int idx = threadIdx.x + 256 * u;
+            const bool live = idx < kChunkF4 && t0 + idx / kTileF4 < t_end;
+            ra[u] = live ? ga[static_cast<size_t>(t0) * kTileF4 + idx] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        if (threadIdx.x < kChunkM4) {
+            const bool live = t0 + static_cast<int>(threadIdx.x) / 8 < t_end;
+            rm = live ? gm[static_cast<size_t>(t0) * 8 + threadIdx.x] : make_float4(nan, nan, nan, nan);
+        }
+    };
+    auto stage_chunk = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < kLoadsA; ++u) {
+            const int idx = threadIdx.x + 256 * u;
+            const int row = idx / kRowF4;
+            if (idx < kChunkF4) s_a[buf * kLdsChunkF4 + row * kPitchF4 + (idx - row * kRowF4)] = ra[u];
+        }
+        if (threadIdx.x < kChunkM4) s_m[buf * kChunkM4 + threadIdx.x] = rm;
+    };
+
+    auto flush = [&]() {
+        // Exact keys of the queued candidates, four at a time so their row
+        // loads overlap, then sorted inserts.
+        for (int r = 0; r < kRing; r += 4) {
+            if (!__any(r < cnt)) break;
+            float e[4];
+            uint32_t id[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool live = r + u < cnt;
+                id[u] = live ? ring[(r + u) * 64 + lane] : 0u;
+                const float* row = p.xs + static_cast<size_t>(id[u]) * DP;
+                if constexpr (kQRegs) {
+                    e[u] = live ? exact_key<DP>(qs, row) : 0.0f;
+                } else {
+                    e[u] = live ? exact_key<DP>(q_global, row) : 0.0f;
+                }
+            }
+            // One copy of the insert network, fed through a shift register so
+            // no register array is indexed at run time.
+            uint64_t w0 = kSentinel, w1 = kSentinel, w2 = kSentinel, w3 = kSentinel;
+            {
+                uint64_t w[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t kb = (static_cast<int>(id[u]) == qi || e[u] != e[u]) ? kInfBits
+                                                                                       : __float_as_uint(e[u]);
+                    w[u] = (r + u < cnt) ? pack(kb, id[u]) : kSentinel;
+                }
+                w0 = w[0];
+                w1 = w[1];
+                w2 = w[2];
+                w3 = w[3];
+            }
+#pragma unroll 1
+            for (int u = 0; u < 4; ++u) {
+                if (w0 < a[0]) list_insert<KMAX>(a, w0);
+                w0 = w1;
+                w1 = w2;
+                w2 = w3;
+            }
+        }
+        cnt = 0;
+        Tn = threshold();
+    };
+
+    // Screen one tile's accumulator against the threshold; queue passes. The
+    // common case (no lane passes) costs a min tree and one compare; when some
+    // lane passes, each register is tested with one wave-wide compare whose
+    // mask is checked on the scalar unit, so registers nobody passes cost no
+    // divergent branch.
+    auto screen = [&](const floatx16& acc, int tile) {
+        float lo = acc[0];
+#pragma unroll
+        for (int r = 1; r < 16; ++r) lo = __builtin_fminf(lo, acc[r]);  // NaN rows drop out
+        if (__any(lo <= Tn)) {
+            if (__any(cnt > kRing - 16)) flush();
+            const uint32_t c0 = static_cast<uint32_t>(tile * 32 + 4 * h);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const bool pass = acc[r] <= Tn;
+                if (__ballot(pass)) {
+                    if (pass) {
+                        ring[cnt * 64 + lane] = c0 + static_cast<uint32_t>(8 * (r / 4) + (r % 4));
+                        ++cnt;
+                    }
+                }
+            }
+        }
+    };
+
+    // Accumulator initialised with the 16 adjusted norms of this lane's rows.
+    auto init_acc = [&](const float4* sm, int tt) {
+        floatx16 acc;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const float4 t = sm[tt * 8 + 2 * b + h];
+            acc[4 * b + 0] = t.x;
+            acc[4 * b + 1] = t.y;
+            acc[4 * b + 2] = t.z;
+            acc[4 * b + 3] = t.w;
+        }
+        return acc;
+    };
+
+    if (n_chunks > 0) {
+        fetch(t_begin);
+        stage_chunk(0);
+    }
+    __syncthreads();
+    for (int ch = 0; ch < n_chunks; ++ch) {
+        const int buf = ch & 1;
+        const int t0 = t_begin + ch * CT;
+        if (ch + 1 < n_chunks) fetch(t0 + CT);  // next chunk in flight
+        const float4* sa = s_a + buf * kLdsChunkF4;
+        const float4* sm = s_m + buf * kChunkM4;
+        // Two tiles per step: two independent MFMA chains hide each other's
+        // result latency.
+#pragma unroll
+        for (int tt = 0; tt < CT; tt += 2) {
+            Frag f0, f1;
+            Op::template load_a<DP>(reinterpret_cast<const Elem*>(sa + (tt * 32 + j) * kPitchF4) +
+                                        Op::half_offset(DP, h), f0);
+            floatx16 acc0 = init_acc(sm, tt);
+            const bool two = tt + 1 < CT;
+            floatx16 acc1;
+            if (two) {
+                Op::template load_a<DP>(reinterpret_cast<const Elem*>(sa + ((tt + 1) * 32 + j) * kPitchF4) +
+                                            Op::half_offset(DP, h), f1);
+                acc1 = init_acc(sm, tt + 1);
+            }
+            acc0 = Op::template run<DP>(f0, bq, acc0);
+            if (two) acc1 = Op::template run<DP>(f1, bq, acc1);
+            if (t0 + tt < t_end) screen(acc0, t0 + tt);
+            if (two && t0 + tt + 1 < t_end) screen(acc1, t0 + tt + 1);
+        }
+        if (ch + 1 < n_chunks) stage_chunk(buf ^ 1);
+        __syncthreads();
+    }
+    flush();
+
+    // ---- fold the two half-lists of each query (lane j + 32 into lane j) and
+    // write the result; the waves take turns on the shared staging bytes ----
+    __syncthreads();  // every ring drained
+    uint64_t* stage = reinterpret_cast<uint64_t*>(smem);
+    for (int w = 0; w < kTilesPerBlock; ++w) {
+        if (wave == w) {
+            if (h == 1) {
+#pragma unroll
+                for (int t = 0; t < KMAX; ++t) stage[t * 64 + lane] = a[t];
+            }
+            wave_lds_sync();
+            if (h == 0) {
+                for (int t = p.k - 1; t >= 0; --t) {
+                    const uint64_t e = stage[t * 64 + lane + 32];
+                    if (e < a[0]) list_insert<KMAX>(a, e);
+                }
+            }
+            wave_lds_sync();
+            if (h == 0 && valid) {
+                if (p.slices > 1) {
+                    // Partial list of this slice, ascending.
+                    uint64_t* dst = p.partial + (static_cast<size_t>(blockIdx.y) * p.rows + local) * p.k;
+#pragma unroll
+                    for (int t = 0; t < KMAX; ++t) {
+                        if (t < p.k) dst[p.k - 1 - t] = a[t];
+                    }
+                } else {
+#pragma unroll
+                    for (int t = 0; t < KMAX; ++t) {
+                        if (t < p.k) stage[(p.k - 1 - t) * 64 + lane] = a[t];
+                    }
+                    if (!emit_list(stage + lane, p.k, n, qi, p.nbr_idx, p.nbr_d2, p.kdist2)) {
+                        p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// One thread per query: fold the slices' partial lists, then emit.
+template <int KMAX>
+__global__ void __launch_bounds__(64)
+merge_slices_kernel(const uint64_t* __restrict__ partial, int slices, int rows, int row0, int n,
+                    int k, int* __restrict__ nbr_idx, float* __restrict__ nbr_d2,
+                    float* __restrict__ kdist2, int* __restrict__ fb_rows, int* __restrict__ fb_count) {
+    __shared__ uint64_t stage[KMAX][64];
+    const int lane = threadIdx.x;
+    const int local = blockIdx.x * 64 + lane;
+    if (local >= rows) return;
+    const int qi = row0 + local;
+    uint64_t a[KMAX];
+    list_init<KMAX>(a, k);
+    for (int s = 0; s < slices; ++s) {
+        const uint64_t* src = partial + (static_cast<size_t>(s) * rows + local) * k;
+        for (int t = 0; t < k; ++t) {
+            const uint64_t e = src[t];
+            if (e >= a[0]) break;  // ascending: the rest cannot enter
+            list_insert<KMAX>(a, e);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < KMAX; ++t) {
+        if (t < k) stage[k - 1 - t][lane] = a[t];
+    }
+    if (!emit_list(&stage[0][lane], k, n, qi, nbr_idx, nbr_d2, kdist2)) {
+        fb_rows[atomicAdd(fb_count, 1)] = qi;
+    }
+}
+
+// ---- operand preparation ----
+
+// Screen operands for rows [0, padded_rows): rows past n are zero.
+__global__ void __launch_bounds__(256)
+screen_rows_f32_kernel(const float* __restrict__ xc, int n, int padded_rows, int dp,
+                       float* __restrict__ xa) {
+    const size_t cell = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (cell >= static_cast<size_t>(padded_rows) * dp) return;
+    const size_t i = cell / dp;
+    const int pos = static_cast<int>(cell % dp);
+    const int half = dp / 2;
+    const int src = pos < half ? 2 * pos : 2 * (pos - half) + 1;
+    xa[cell] = i < static_cast<size_t>(n) ? xc[i * dp + src] : 0.0f;
+}
+
+__global__ void __launch_bounds__(256)
+screen_rows_bf16_kernel(const float* __restrict__ xc, size_t cells, size_t padded_cells,
+                        __bf16* __restrict__ xb) {
+    const size_t cell = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (cell < padded_cells) xb[cell] = static_cast<__bf16>(cell < cells ? xc[cell] : 0.0f);
+}
+
+// |x_i|^2 (1 - c) for rows [0, padded_rows); NaN past n, which fails every compare.
+__global__ void __launch_bounds__(256)
+adjusted_norms_kernel(const float* __restrict__ norms, int n, int padded_rows, float c,
+                      float* __restrict__ madj) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < padded_rows) madj[i] = i < n ? norms[i] * (1.0f - c) : __uint_as_float(0x7FC00000u);
+}
+
+template <class Op, int DP, int KMAX>
+void launch_mfma(const MfmaArgs& a, hipStream_t stream) {
+    const int qtiles = (a.rows + 31) / 32;
+    const dim3 grid((qtiles + kTilesPerBlock - 1) / kTilesPerBlock, a.slices);
+    {
+        ProfileScope prof("mfma_sweep", stream);
+        mfma_sweep_kernel<Op, DP, KMAX><<<grid, 64 * kTilesPerBlock, 0, stream>>>(a);
+    }
+    if (a.slices > 1) {
+        ProfileScope prof("merge_slices", stream);
+        merge_slices_kernel<KMAX><<<(a.rows + 63) / 64, 64, 0, stream>>>(
+            a.partial, a.slices, a.rows, a.row0, a.n, a.k, a.nbr_idx, a.nbr_d2, a.kdist2,
+            a.fb_rows, a.fb_count);
+    }
+}
+
+template <class Op, int DP>
+bool dispatch_k(const MfmaArgs* a, int k, hipStream_t stream) {
+    if (k <= 16) {
+        if (a) launch_mfma<Op, DP, 16>(*a, stream);
+        return true;
+    }
+    if (k <= 32) {
+        if (a) launch_mfma<Op, DP, 32>(*a, stream);
+        return true;
+    }
+    if (k <= 64) {
+        if (a) launch_mfma<Op, DP, 64>(*a, stream);
+        return true;
+    }
+    return false;
+}
+
+template <class Op>
+bool dispatch(int dp, int k, const MfmaArgs* a, hipStream_t stream) {
+    if constexpr (std::is_same_v<Op, ScreenF32>) {
+        switch (dp) {
+            case 4: return dispatch_k<Op, 4>(a, k, stream);
+            case 8: return dispatch_k<Op, 8>(a, k, stream);
+            case 16: return dispatch_k<Op, 16>(a, k, stream);
+            case 32: return dispatch_k<Op, 32>(a, k, stream);
+            case 64: return dispatch_k<Op, 64>(a, k, stream);
+            case 128: return dispatch_k<Op, 128>(a, k, stream);
+            default: return false;
+        }
+    } else {
+        switch (dp) {
+            case 16: return dispatch_k<Op, 16>(a, k, stream);
+            case 32: return dispatch_k<Op, 32>(a, k, stream);
+            case 64: return dispatch_k<Op, 64>(a, k, stream);
+            case 128: return dispatch_k<Op, 128>(a, k, stream);
+            default: return false;
+        }
+    }
+}
+
+}  // namespace
+
+int mfma_padded_dims(int n_dims, int form) {
+    const int dp = padded_dims(n_dims);
+    return (form == kFormBF16 && dp < 16) ? 16 : dp;
+}
+
+bool mfma_supported(int n_dims, int k, int form) {
+    const int dp = mfma_padded_dims(n_dims, form);
+    return form == kFormBF16 ? dispatch<ScreenBF16>(dp, k, nullptr, nullptr)
+                             : dispatch<ScreenF32>(dp, k, nullptr, nullptr);
+}
+
+int mfma_slices(int n_points, int rows) {
+    // Each slice restarts every query's list, so split the candidates only as
+    // far as needed for about 8 query tiles per SIMD (1024 SIMDs), and never
+    // below ~96 candidate tiles per slice.
+    const long long waves = (rows + 31) / 32;
+    long long s = (8192 + waves - 1) / waves;
+    const long long max_by_len = std::max(1, ((n_points + 31) / 32) / 96);
+    s = std::min<long long>(std::min<long long>(s, 8), max_by_len);
+    return static_cast<int>(std::max<long long>(s, 1));
+}
+
+size_t mfma_scratch_bytes(int n_points, int dp, int form, int rows, int k) {
+    const size_t padded = static_cast<size_t>((n_points + 31) / 32) * 32;
+    const size_t rows_bytes = form == kFormBF16 ? sizeof(uint16_t) : sizeof(float);
+    const size_t operand = padded * dp * rows_bytes;
+    const size_t madj = sizeof(float) * padded;
+    const int s = mfma_slices(n_points, rows);
+    const size_t partial = s > 1 ? sizeof(uint64_t) * s * static_cast<size_t>(rows) * k : 0;
+    return operand + 256 + madj + 256 + partial + 256;
+}
+
+void mfma_knn(const float* keyrows, const float* centred, int stride, const float* norms,
+              const int* guard, int n_points, int n_dims, int k, int form, int row0, int rows,
+              void* scratch, int* nbr_idx, float* nbr_d2, float* kdist2, int* fallback_rows,
+              int* fallback_count, hipStream_t stream) {
+    const int dp = mfma_padded_dims(n_dims, form);
+    if (stride != dp) throw std::invalid_argument("mfma_knn: row stride mismatch");
+    if (rows <= 0) return;
+    const bool bf16 = form == kFormBF16;
+    const float c = bf16 ? ScreenBF16::slack(dp) : ScreenF32::slack(dp);
+    const int padded_rows = (n_points + 31) / 32 * 32;
+    const size_t cells = static_cast<size_t>(n_points) * dp;
+    const size_t padded_cells = static_cast<size_t>(padded_rows) * dp;
+    unsigned char* base = static_cast<unsigned char*>(scratch);
+    void* xa = base;
+    base += (padded_cells * (bf16 ? 2 : 4) + 255) / 256 * 256;
+    float* madj = reinterpret_cast<float*>(base);
+    base += (sizeof(float) * padded_rows + 255) / 256 * 256;
+    uint64_t* partial = reinterpret_cast<uint64_t*>(base);
+
+    {
+        ProfileScope prof("screen_prep", stream);
+        const unsigned grid = static_cast<unsigned>((padded_cells + 255) / 256);
+        if (bf16) {
+            screen_rows_bf16_kernel<<<grid, 256, 0, stream>>>(centred, cells, padded_cells,
+                                                                static_cast<__bf16*>(xa));
+        } else {
+            screen_rows_f32_kernel<<<grid, 256, 0, stream>>>(centred, n_points, padded_rows, dp,
+                                                               static_cast<float*>(xa));
+        }
+        CULOF_CHECK(hipGetLastError());
+        adjusted_norms_kernel<<<(padded_rows + 255) / 256, 256, 0, stream>>>(norms, n_points,
+                                                                              padded_rows, c, madj);
+        CULOF_CHECK(hipGetLastError());
+    }
+    const MfmaArgs args{xa, centred, keyrows, madj, norms, guard, n_points, k, c, row0, rows,
+                        mfma_slices(n_points, rows), partial, nbr_idx, nbr_d2, kdist2,
+                        fallback_rows, fallback_count};
+    const bool ok = bf16 ? dispatch<ScreenBF16>(dp, k, &args, stream)
+                         : dispatch<ScreenF32>(dp, k, &args, stream);
+    if (!ok) throw std::invalid_argument("mfma_knn: unsupported (n_dims, k, form)");
+    CULOF_CHECK(hipGetLastError());
+}
+
+}  // namespace detail
+}  // namespace culof

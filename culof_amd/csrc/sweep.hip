@@ -55,7 +55,8 @@ namespace {
 
 constexpr uint64_t kSentinel = ~0ull;  // above every real (key, index) word
 constexpr int kWaves = 4;              // waves per block, all on the same 64 queries
-constexpr int kRing = 8;               // ring entries per lane
+constexpr int kGroup = 8;              // candidates screened per branch
+constexpr int kRing = 16;              // ring entries per lane (> kGroup)
 
 __device__ __forceinline__ uint64_t pack(uint32_t key, uint32_t idx) {
     return (static_cast<uint64_t>(key) << 32) | idx;
@@ -254,53 +255,65 @@ sweep_kernel(const float* __restrict__ xs, const float* __restrict__ xc,
         if (m + kWaves < n_chunks) base = load_chunk(m + kWaves);  // next chunk in flight
         refresh_bound();
 
-        for (int c = 0; c < CH; c += 4) {
-            if (__any(cnt > kRing - 4)) {
+        constexpr int G = CH < kGroup ? CH : kGroup;
+        for (int c = 0; c < CH; c += G) {
+            if (__any(cnt > kRing - G)) {
                 merge_ring();
                 refresh_bound();
             }
+            // Screen G candidates branch-free: G independent FMA chains whose
+            // LDS reads can all be in flight together.
+            float raw[G];
 #pragma unroll
-            for (int cc = 0; cc < 4; ++cc) {
-                const float4* cv = s_cc + (c + cc) * V4;
-                float g = 0.0f;
+            for (int g = 0; g < G; ++g) {
+                const float4* cv = s_cc + (c + g) * V4;
+                float acc = 0.0f;
 #pragma unroll
                 for (int v = 0; v < V4; ++v) {
                     const float4 t = cv[v];
-                    g = __builtin_fmaf(qc[4 * v + 0], t.x, g);
-                    g = __builtin_fmaf(qc[4 * v + 1], t.y, g);
-                    g = __builtin_fmaf(qc[4 * v + 2], t.z, g);
-                    g = __builtin_fmaf(qc[4 * v + 3], t.w, g);
+                    acc = __builtin_fmaf(qc[4 * v + 0], t.x, acc);
+                    acc = __builtin_fmaf(qc[4 * v + 1], t.y, acc);
+                    acc = __builtin_fmaf(qc[4 * v + 2], t.z, acc);
+                    acc = __builtin_fmaf(qc[4 * v + 3], t.w, acc);
                 }
-                const float raw = __builtin_fmaf(-2.0f, g, nq + s_cm[c + cc]);
-                if (raw <= Tq) {
-                    // Exact key: difference form over the key rows.
-                    const int j = cur_base + c + cc;
-                    const float4* sv = s_cs + (c + cc) * V4;
-                    const float4* qv = reinterpret_cast<const float4*>(xs + static_cast<size_t>(qi) * DP);
-                    float e = 0.0f;
+                raw[g] = __builtin_fmaf(-2.0f, acc, nq + s_cm[c + g]);
+            }
+            float lo = raw[0];
 #pragma unroll
-                    for (int v = 0; v < V4; ++v) {
-                        const float4 t = sv[v];
-                        float4 q4;
-                        if constexpr (Shape::kKeyRowInRegs) {
-                            q4 = make_float4(qs[4 * v + 0], qs[4 * v + 1], qs[4 * v + 2], qs[4 * v + 3]);
-                        } else {
-                            q4 = qv[v];
+            for (int g = 1; g < G; ++g) lo = fminf(lo, raw[g]);  // NaN (past the end) drops out
+            if (lo <= Tq) {
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    if (raw[g] <= Tq) {
+                        // Exact key: difference form over the key rows.
+                        const int j = cur_base + c + g;
+                        const float4* sv = s_cs + (c + g) * V4;
+                        const float4* qv = reinterpret_cast<const float4*>(xs + static_cast<size_t>(qi) * DP);
+                        float e = 0.0f;
+#pragma unroll
+                        for (int v = 0; v < V4; ++v) {
+                            const float4 t = sv[v];
+                            float4 q4;
+                            if constexpr (Shape::kKeyRowInRegs) {
+                                q4 = make_float4(qs[4 * v + 0], qs[4 * v + 1], qs[4 * v + 2], qs[4 * v + 3]);
+                            } else {
+                                q4 = qv[v];
+                            }
+                            float df = q4.x - t.x;
+                            e = __builtin_fmaf(df, df, e);
+                            df = q4.y - t.y;
+                            e = __builtin_fmaf(df, df, e);
+                            df = q4.z - t.z;
+                            e = __builtin_fmaf(df, df, e);
+                            df = q4.w - t.w;
+                            e = __builtin_fmaf(df, df, e);
                         }
-                        float df = q4.x - t.x;
-                        e = __builtin_fmaf(df, df, e);
-                        df = q4.y - t.y;
-                        e = __builtin_fmaf(df, df, e);
-                        df = q4.z - t.z;
-                        e = __builtin_fmaf(df, df, e);
-                        df = q4.w - t.w;
-                        e = __builtin_fmaf(df, df, e);
-                    }
-                    const uint32_t kb = (j == qi || e != e) ? kInfBits : __float_as_uint(e);
-                    const uint64_t p = pack(kb, static_cast<uint32_t>(j));
-                    if (p < a[0]) {
-                        s_ring[cnt * 64 + lane] = p;
-                        ++cnt;
+                        const uint32_t kb = (j == qi || e != e) ? kInfBits : __float_as_uint(e);
+                        const uint64_t p = pack(kb, static_cast<uint32_t>(j));
+                        if (p < a[0]) {
+                            s_ring[cnt * 64 + lane] = p;
+                            ++cnt;
+                        }
                     }
                 }
             }

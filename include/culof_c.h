@@ -37,9 +37,12 @@ enum culof_status {
     CULOF_NO_DEVICE = 3         /* CudaError("no ... device available") */
 };
 
-/* Precision forms of the distance stage. */
+/* Precision forms of the distance screen. Selection is always by the exact
+ * FP32 difference-form keys, so both forms return the same neighbours, the
+ * same k-distances and the same scores; they differ in speed only. */
 enum culof_form {
-    CULOF_FORM_PLAIN = 0 /* FP32 keys throughout */
+    CULOF_FORM_PLAIN = 0,      /* FP32 Gram screen (MFMA or VALU) */
+    CULOF_FORM_BF16_SCREEN = 1 /* BF16 MFMA screen, wider error margin */
 };
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -86,13 +89,14 @@ int culof_stage_center(const float* points, int n_points, int n_dims, int normal
  * `points` (centred, and z-scored with `normalize`, internally). Keys are
  * difference-form FP32 squared distances sum_t (a_t - b_t)^2 of the input rows
  * (of the z-scored rows with `normalize`), +inf on the diagonal; ties go to
- * the lower index. method 0 = fused Gram-screened sweep (the production path),
- * method 1 = per-row radix select recomputing keys every pass; both return the
- * same bits. Outputs use stride n_points (slot t of point i at
+ * the lower index. method 0 = the production path for `form` (a screened
+ * sweep on the matrix cores where instantiated), 1 = per-row radix select
+ * recomputing keys every pass, 2 = the VALU-screened sweep (form 0 only); all
+ * return the same bits. Outputs use stride n_points (slot t of point i at
  * [t * n_points + i]). Replaces the SGEMM + detail::select_knn tile step
  * (src/cuda/pipeline.cu:175-195, include/culof_internal.cuh:52). */
 int culof_stage_knn(const float* points, int n_points, int n_dims, int k, int normalize,
-                    int row0, int rows, int method, int* nbr_idx, float* nbr_d2,
+                    int form, int row0, int rows, int method, int* nbr_idx, float* nbr_d2,
                     float* kdist2, void* stream);
 
 /* Exact radix select over stored key rows: keys is row-major (rows x n_cols)

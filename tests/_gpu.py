@@ -60,7 +60,7 @@ def center(X, normalize=False):
     return xc.cpu().numpy(), norms.cpu().numpy()
 
 
-def knn(X, k, normalize=False, row0=0, rows=None, method=0):
+def knn(X, k, normalize=False, row0=0, rows=None, method=0, form=_native.FORM_PLAIN):
     X = np.ascontiguousarray(X, np.float32)
     n, d = X.shape
     rows = n - row0 if rows is None else rows
@@ -69,7 +69,7 @@ def knn(X, k, normalize=False, row0=0, rows=None, method=0):
     d2 = torch.full((k, n), float("nan"), dtype=torch.float32, device=DEV)
     kd = torch.full((n,), float("nan"), dtype=torch.float32, device=DEV)
     _native.check(_native.lib().culof_stage_knn(
-        x.data_ptr(), n, d, k, int(normalize), row0, rows, method, idx.data_ptr(), d2.data_ptr(),
+        x.data_ptr(), n, d, k, int(normalize), form, row0, rows, method, idx.data_ptr(), d2.data_ptr(),
         kd.data_ptr(), _native.current_stream()))
     _sync()
     sl = slice(row0, row0 + rows)

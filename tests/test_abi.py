@@ -105,7 +105,7 @@ def test_device_entry_points_validate_before_the_device():
     assert rc == _native.CULOF_INVALID_ARGUMENT
     rc = lib.culof_stage_select_rows(None, 1, 1, 1, None, None, None, None)
     assert rc == _native.CULOF_INVALID_ARGUMENT
-    rc = lib.culof_stage_knn(None, 10, 2, 3, 0, 0, 10, 0, None, None, None, None)
+    rc = lib.culof_stage_knn(None, 10, 2, 3, 0, 0, 0, 10, 0, None, None, None, None)
     assert rc == _native.CULOF_INVALID_ARGUMENT
 
 

@@ -84,18 +84,51 @@ def test_center_bit_exact(shape, normalize):
 
 @pytest.mark.parametrize("normalize", [False, True])
 @pytest.mark.parametrize("shape", [s for s in SHAPES if s[1] <= 128 and s[2] <= 64], ids=_shape_id)
-def test_sweep_matches_radix_and_oracle(shape, normalize):
+def test_every_path_matches_oracle(shape, normalize):
+    """The matrix-core sweep (both screen forms), the VALU sweep and the
+    radix path all return the oracle's bits."""
     from _gpu import knn
 
     n, d, k, n_out, seed = shape
     X = make_data(n, d, seed, n_out)
-    sw = knn(X, k, normalize=normalize, method=0)
-    rx = knn(X, k, normalize=normalize, method=1)
     oi, od, ok = oracle.knn_rows(oracle.key_rows_of(X, normalize), k)
-    for got, label in ((sw, "sweep"), (rx, "radix")):
+    runs = {
+        "mfma-fp32": knn(X, k, normalize=normalize, method=0, form=0),
+        "mfma-bf16": knn(X, k, normalize=normalize, method=0, form=1),
+        "valu": knn(X, k, normalize=normalize, method=2),
+        "radix": knn(X, k, normalize=normalize, method=1),
+    }
+    for label, got in runs.items():
         np.testing.assert_array_equal(got[0].T, oi, err_msg=f"{label} nbr_idx")
         _assert_same_bits(got[1].T, od, f"{label} nbr_d2")
         _assert_same_bits(got[2], ok, f"{label} kdist2")
+
+
+@pytest.mark.parametrize("form", [0, 1])
+@pytest.mark.parametrize("shape", [(5000, 8, 20, 50, 51), (6000, 32, 50, 60, 52), (4100, 128, 20, 41, 53),
+                                   (7000, 3, 8, 70, 54)], ids=_shape_id)
+def test_sliced_sweep_matches_oracle(shape, form):
+    """Sizes where the candidate range is split into slices and merged."""
+    from _gpu import knn
+
+    n, d, k, n_out, seed = shape
+    X = make_data(n, d, seed, n_out)
+    got = knn(X, k, method=0, form=form)
+    rows = np.r_[0:200, n // 2:n // 2 + 100, n - 200:n]
+    oi, od, ok = oracle.knn_rows(X, k)
+    np.testing.assert_array_equal(got[0].T, oi)
+    _assert_same_bits(got[2], ok, "kdist2")
+
+
+def test_forms_give_identical_forward():
+    from _gpu import forward
+    from culof_amd import workloads
+
+    X = workloads.lowrank(n=3000, d=128, seed=2)
+    plain = forward(X, 50, form=0)
+    screened = forward(X, 50, form=1)
+    for key in plain:
+        np.testing.assert_array_equal(plain[key], screened[key], err_msg=key)
 
 
 def test_partial_row_ranges():
