@@ -20,6 +20,11 @@ torch.cuda.synchronize(). Prints one JSON line:
                full workload (the reference's own comparator), with agreement
 * parity       the sampled rows' neighbour lists and k-distances compared bit
                for bit with the oracle, and scores against scikit-learn
+* fwd_bwd      forward + hand-written backward of sum(w * LOF) at n=50k, d=32,
+               k=20 (points/s of both passes, and the ratio to the forward)
+* forms        the BF16-screen form against the plain FP32 form at n=100k,
+               d=128, k=50: points/s of each and whether the kNN sets, the
+               k-distances and the scores are identical
 """
 
 from __future__ import annotations
@@ -59,6 +64,9 @@ def parse_args():
     p.add_argument("--cpu-rows", type=int, default=2000,
                    help="query rows in the CPU-oracle sample (0 skips the leg)")
     p.add_argument("--no-sklearn", action="store_true", help="skip the scikit-learn leg")
+    p.add_argument("--no-forms", action="store_true",
+                   help="skip the fwd+bwd (n=50k, d=32, k=20) and bf16-vs-fp32 (n=100k, d=128, "
+                        "k=50) legs")
     p.add_argument("--calibrate-copy-mib", type=int, default=0,
                    help="only run a plain device copy of this many MiB three times (the "
                         "known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -87,11 +95,84 @@ def read_traffic(workload: str):
     return float(m.group(1)) if m else None
 
 
+def _time_device(fn, steps: int) -> float:
+    """Seconds per call of ``fn`` over ``steps`` calls after one untimed call."""
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def forms_legs(seed: int, steps: int) -> dict:
+    """The differentiation form and the precision form, each on its own workload."""
+    import torch
+
+    from culof_amd import workloads
+    from culof_amd.autograd import backward_points, forward_state
+
+    out = {}
+    X, w = workloads.make("grad-50k", seed=seed)
+    n, d = X.shape
+    k = workloads.WORKLOADS["grad-50k"]["k"]
+    x = torch.from_numpy(X).cuda()
+    wt = torch.from_numpy(w).cuda()
+    state = {}
+
+    def fwd():
+        state["s"] = forward_state(x, k)
+
+    def fwd_bwd():
+        st = forward_state(x, k)
+        backward_points(x, st, wt)
+
+    t_f = _time_device(fwd, steps)
+    t_fb = _time_device(fwd_bwd, steps)
+    g1 = backward_points(x, state["s"], wt)
+    g2 = backward_points(x, state["s"], wt)
+    out["fwd_bwd"] = {
+        "workload": "grad-50k", "n": n, "d": d, "k": k,
+        "fwd_points_per_s": round(n / t_f, 2),
+        "fwd_bwd_points_per_s": round(n / t_fb, 2),
+        "ms_fwd": round(t_f * 1e3, 4), "ms_fwd_bwd": round(t_fb * 1e3, 4),
+        "fwd_bwd_over_fwd": round(t_fb / t_f, 3),
+        "backward_bit_identical_repeat": bool(torch.equal(g1, g2)),
+        "grad_finite": bool(torch.isfinite(g1).all()),
+    }
+    log(f"[bench] fwd+bwd {out['fwd_bwd']}")
+    del x, wt, state, g1, g2
+
+    X = workloads.make("bf16-100k", seed=seed)
+    n, d = X.shape
+    k = workloads.WORKLOADS["bf16-100k"]["k"]
+    x = torch.from_numpy(X).cuda()
+    res = {}
+    for form in ("fp32", "bf16"):
+        t = _time_device(lambda f=form: res.__setitem__(f, forward_state(x, k, form=f)), steps)
+        res[form + "_s"] = t
+    same_idx = bool(torch.equal(res["fp32"].idx, res["bf16"].idx))
+    same_kd = bool(torch.equal(res["fp32"].kdist2, res["bf16"].kdist2))
+    same_scores = bool(torch.equal(res["fp32"].scores, res["bf16"].scores))
+    out["forms"] = {
+        "workload": "bf16-100k", "n": n, "d": d, "k": k,
+        "fp32_points_per_s": round(n / res["fp32_s"], 2),
+        "bf16_points_per_s": round(n / res["bf16_s"], 2),
+        "bf16_over_fp32": round(res["fp32_s"] / res["bf16_s"], 3),
+        "identical_knn": same_idx and same_kd, "identical_scores": same_scores,
+    }
+    log(f"[bench] forms {out['forms']}")
+    return out
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the path shards by query rows, see DESIGN.md)", file=sys.stderr)
+              "(the path would shard by query rows; this measures one GPU's share)", file=sys.stderr)
         return 2
 
     if args.calibrate_copy_mib:
@@ -224,6 +305,8 @@ def main() -> int:
         parity["vs_sklearn_top1pct_agreement"] = agree
         log(f"[bench] sklearn {sk_s:.2f} s; agreement {parity}")
 
+    extra = {} if args.no_forms else forms_legs(args.seed, max(2, min(args.steps, 5)))
+
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -244,6 +327,7 @@ def main() -> int:
         "sklearn": sklearn_leg,
         "kernels": kernels,
         "parity": parity,
+        **extra,
         "baseline_note": "vs_baseline divides by the reference's published 1.42e5 points/s "
                          "(RTX A6000, host arrays, best of 7)",
     }

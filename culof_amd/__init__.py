@@ -95,7 +95,7 @@ def _check_representable(arr: NDArray[np.float32], n_features: int) -> None:
         )
 
 
-def lof(X: ArrayLike, k: int = 20, normalize: bool = False) -> NDArray[np.float32]:
+def lof(X: ArrayLike, k: int = 20, normalize: bool = False, form: str = "fp32") -> NDArray[np.float32]:
     """Local Outlier Factor of every row of ``X`` (reference: src/culof/__init__.py:83-113).
 
     Parameters
@@ -107,6 +107,10 @@ def lof(X: ArrayLike, k: int = 20, normalize: bool = False) -> NDArray[np.float3
     normalize : bool, default=False
         Z-score each feature before computing distances. scikit-learn does not,
         so leave it off when comparing.
+    form : {"fp32", "bf16"}, default="fp32"
+        Precision form of the distance screen. ``"bf16"`` screens candidates
+        with BF16 matrix-core products and admits them on exact FP32 distances;
+        it returns the same scores as ``"fp32"``. (Not in the reference.)
 
     Returns
     -------
@@ -114,7 +118,14 @@ def lof(X: ArrayLike, k: int = 20, normalize: bool = False) -> NDArray[np.float3
         About 1.0 for a point as densely surrounded as its neighbours, larger
         for one in a comparatively sparse region.
     """
-    scores = _culof.lof(_validate(X, k), k, normalize)
+    arr = _validate(X, k)
+    if form == "fp32":
+        scores = _culof.lof(arr, k, normalize)
+    else:
+        from .autograd import form_code, forward_state
+
+        code = form_code(form)
+        scores = forward_state(torch.from_numpy(arr).cuda(), k, normalize, code).scores.cpu().numpy()
     # A non-finite score would read as "not an outlier" downstream.
     if not np.isfinite(scores).all():
         raise ValueError(
@@ -123,6 +134,15 @@ def lof(X: ArrayLike, k: int = 20, normalize: bool = False) -> NDArray[np.float3
             "normalize=True."
         )
     return scores
+
+
+def __getattr__(name: str):
+    # The autograd binding needs PyTorch-ROCm; it loads on first use.
+    if name in ("lof_torch", "LOFFunction"):
+        from . import autograd
+
+        return getattr(autograd, name)
+    raise AttributeError(f"module {__name__!r} has no attribute {name!r}")
 
 
 class LOF:
@@ -142,6 +162,8 @@ class LOF:
         uses scikit-learn's threshold of -1.5.
     normalize : bool, default=False
         Z-score each feature before computing distances.
+    form : {"fp32", "bf16"}, default="fp32"
+        Precision form of the distance screen; both give the same scores.
 
     Attributes
     ----------
@@ -158,6 +180,7 @@ class LOF:
         n_neighbors: int = 20,
         contamination: str | float = "auto",
         normalize: bool = False,
+        form: str = "fp32",
     ) -> None:
         if n_neighbors < 1:
             raise ValueError(f"n_neighbors must be >= 1, got {n_neighbors}")
@@ -166,11 +189,12 @@ class LOF:
         self.n_neighbors = n_neighbors
         self.contamination = contamination
         self.normalize = normalize
+        self.form = form
 
     def fit(self, X: ArrayLike, y: object = None) -> LOF:
         """Compute the LOF of every sample in ``X``. Returns ``self``."""
         del y  # accepted for scikit-learn call compatibility
-        scores = lof(X, self.n_neighbors, self.normalize)
+        scores = lof(X, self.n_neighbors, self.normalize, self.form)
         self.n_neighbors_ = self.n_neighbors
         self.negative_outlier_factor_ = -scores
         if self.contamination == "auto":
@@ -210,7 +234,8 @@ class LOF:
         return self.offset_
 
     def __repr__(self) -> str:
+        form = "" if self.form == "fp32" else f", form={self.form!r}"
         return (
             f"LOF(n_neighbors={self.n_neighbors}, "
-            f"contamination={self.contamination!r}, normalize={self.normalize})"
+            f"contamination={self.contamination!r}, normalize={self.normalize}{form})"
         )

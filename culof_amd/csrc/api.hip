@@ -191,7 +191,7 @@ Prepared prepare(const float* points, int n, int d, bool normalize, int form, Sc
     const size_t cells = static_cast<size_t>(n) * p.stride;
     float* xc = scratch.get<float>(cells);
     float* norms = scratch.get<float>(n);
-    double* work = scratch.get<double>(static_cast<size_t>(258) * d);
+    double* work = scratch.get<double>(static_cast<size_t>(kPrepWork) * d);
     int* guard = scratch.get<int>(1);
     float* xs = normalize ? nullptr : scratch.get<float>(cells);
     CULOF_CHECK(hipMemsetAsync(guard, 0, sizeof(int), s));
@@ -412,7 +412,7 @@ int culof_stage_center(const float* points, int n_points, int n_dims, int normal
         require_device();
         const hipStream_t s = as_stream(stream);
         Scratch scratch(s);
-        double* work = scratch.get<double>(static_cast<size_t>(258) * n_dims);
+        double* work = scratch.get<double>(static_cast<size_t>(kPrepWork) * n_dims);
         center_and_norms(points, n_points, n_dims, normalize != 0, centred, n_dims, norms, work,
                          nullptr, nullptr, s);
     });
@@ -437,6 +437,37 @@ int culof_stage_knn(const float* points, int n_points, int n_dims, int k, int no
         const Prepared p = prepare(points, n_points, n_dims, normalize != 0, form, scratch, s);
         knn_rows(p, n_points, n_dims, k, form, row0, rows, method, nbr_idx, nbr_d2, kdist2,
                  scratch, s);
+    });
+}
+
+int culof_lof_backward_device(const float* points, int n_points, int n_dims, int k,
+                              int normalize, const int* nbr_idx, const float* nbr_d2,
+                              const float* kdist2, const float* grad_scores, float* grad_points,
+                              void* stream) {
+    return guarded([&] {
+        validate(points, n_points, n_dims, k);
+        if (!nbr_idx || !nbr_d2 || !kdist2 || !grad_scores || !grad_points)
+            throw std::invalid_argument("inputs and outputs must not be null");
+        if (n_dims > kMaxBackwardDims) throw std::invalid_argument("backward supports n_dims <= 256");
+        require_device();
+        const hipStream_t s = as_stream(stream);
+        Scratch scratch(s);
+        const float* keyrows = points;
+        const double* inv_std = nullptr;
+        const double* scaled = nullptr;
+        if (normalize) {
+            // The z-scored rows and the per-feature scale, exactly as the forward built them.
+            float* z = scratch.get<float>(static_cast<size_t>(n_points) * n_dims);
+            float* norms = scratch.get<float>(n_points);
+            double* work = scratch.get<double>(static_cast<size_t>(kPrepWork) * n_dims);
+            center_and_norms(points, n_points, n_dims, true, z, n_dims, norms, work, nullptr, nullptr, s);
+            keyrows = z;
+            inv_std = work + static_cast<size_t>(257) * n_dims;
+            scaled = work + static_cast<size_t>(258) * n_dims;
+        }
+        void* ws = scratch.get<unsigned char>(backward_scratch_bytes(n_points, n_dims, k));
+        lof_backward(keyrows, n_points, n_dims, k, nbr_idx, nbr_d2, kdist2, grad_scores, inv_std,
+                     scaled, grad_points, ws, s);
     });
 }
 

@@ -24,6 +24,9 @@ void check_hip(hipError_t err, const char* file, int line, const char* expr);
 // Bit pattern of +inf: the key of a point against itself and of a NaN expansion.
 constexpr uint32_t kInfBits = 0x7F800000u;
 
+constexpr int kPrepWork = 256 + 3;  // doubles per feature of center_and_norms' work buffer
+constexpr int kMaxBackwardDims = 256;
+
 // Precision forms of the distance screen (culof_form in culof_c.h).
 constexpr int kFormPlain = 0;  // FP32 screen
 constexpr int kFormBF16 = 1;   // BF16 MFMA screen
@@ -46,7 +49,8 @@ private:
 
 // ---- prep.hip: FP64 centring and norms (reference src/cuda/preprocess.cu) ----
 // Writes `centred` with row stride `stride` (>= n_dims, padding zeroed) and
-// FP64-accumulated squared norms. `work` must hold (256 + 2) * n_dims doubles.
+// FP64-accumulated squared norms. `work` must hold kPrepWork * n_dims doubles:
+// partials (256 per feature), then mean, inv_std and scaled (1.0 / 0.0) per feature.
 // Optional: `copy` receives the input rows at the same stride (zero padding);
 // *guard (zeroed by the caller) is set when some norm exceeds FLT_MAX / 16.
 void center_and_norms(const float* points, int n_points, int n_dims, bool scale, float* centred,
@@ -95,6 +99,15 @@ void radix_knn(const float* keyrows, int stride, int n_points, int n_dims, int k
 // Stored mode: keys row-major (rows x n_cols); outputs with stride `rows`.
 void radix_select_rows(const float* keys, int rows, int n_cols, int k, int* nbr_idx,
                        float* nbr_d2, float* kdist2, hipStream_t stream);
+
+// ---- backward.hip: d(sum w LOF)/dX with kNN sets and max() branches fixed ----
+size_t backward_scratch_bytes(int n, int d, int k);
+// keyrows: n x d (stride d) rows the forward keyed on. inv_std / scaled (per
+// feature, from center_and_norms' work buffer) are given when the forward
+// normalised, null otherwise.
+void lof_backward(const float* keyrows, int n, int d, int k, const int* nbr_idx, const float* nbr_d2,
+                  const float* kdist2, const float* grad_scores, const double* inv_std,
+                  const double* scaled, float* grad_points, void* scratch, hipStream_t s);
 
 // ---- density.hip: reference src/cuda/lof.cu ----
 void compute_lrd(const int* nbr_idx, const float* nbr_d2, const float* kdist2, int n_points,

@@ -61,10 +61,12 @@ column_partials_kernel(const float* __restrict__ x, int n, int d, const double* 
 }
 
 // One block per feature: halving tree over the 256 partials.
-// pass 0 writes mean = sum / n; pass 1 writes inv_std from the variance.
+// pass 0 writes mean = sum / n; pass 1 writes inv_std from the variance and
+// flags whether the feature was scaled (1.0) or left centred (0.0).
 __global__ void __launch_bounds__(256)
 column_finalize_kernel(const double* __restrict__ part, int n, int d, int pass,
-                       double* __restrict__ mean, double* __restrict__ inv_std) {
+                       double* __restrict__ mean, double* __restrict__ inv_std,
+                       double* __restrict__ scaled) {
     const int f = blockIdx.x;
     __shared__ double s[kLanes];
     s[threadIdx.x] = part[static_cast<size_t>(threadIdx.x) * d + f];
@@ -77,10 +79,12 @@ column_finalize_kernel(const double* __restrict__ part, int n, int d, int pass,
         if (pass == 0) {
             mean[f] = s[0] / n;
             inv_std[f] = 1.0;
+            scaled[f] = 0.0;
         } else {
             const double sd = sqrt(s[0] / n);
             // A constant feature stays centred at zero rather than divided by noise.
             inv_std[f] = (sd > 1e-12) ? 1.0 / sd : 1.0;
+            scaled[f] = (sd > 1e-12) ? 1.0 : 0.0;
         }
     }
 }
@@ -123,6 +127,7 @@ void center_and_norms(const float* points, int n_points, int n_dims, bool scale,
     double* part = work;                                   // 256 x d
     double* mean = work + static_cast<size_t>(kLanes) * n_dims;
     double* inv_std = mean + n_dims;
+    double* scaled = inv_std + n_dims;
     const int threads = kLanes * n_dims;
     const int grid = (threads + 255) / 256;
     {
@@ -130,13 +135,14 @@ void center_and_norms(const float* points, int n_points, int n_dims, bool scale,
         column_partials_kernel<<<grid, 256, 0, stream>>>(points, n_points, n_dims, nullptr, part);
         CULOF_CHECK(hipGetLastError());
     }
-    column_finalize_kernel<<<n_dims, kLanes, 0, stream>>>(part, n_points, n_dims, 0, mean, inv_std);
+    column_finalize_kernel<<<n_dims, kLanes, 0, stream>>>(part, n_points, n_dims, 0, mean, inv_std,
+                                                          scaled);
     CULOF_CHECK(hipGetLastError());
     if (scale) {
         column_partials_kernel<<<grid, 256, 0, stream>>>(points, n_points, n_dims, mean, part);
         CULOF_CHECK(hipGetLastError());
         column_finalize_kernel<<<n_dims, kLanes, 0, stream>>>(part, n_points, n_dims, 1, mean,
-                                                              inv_std);
+                                                              inv_std, scaled);
         CULOF_CHECK(hipGetLastError());
     }
     {

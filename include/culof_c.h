@@ -73,6 +73,19 @@ int culof_lof_device(const float* points, int n_points, int n_dims, int k, int n
                      int form, float* scores, int* nbr_idx, float* nbr_d2, float* kdist2,
                      float* lrd, void* stream);
 
+/* Backward of the device-resident forward: grad_points (n_points x n_dims,
+ * device) = d(sum_i grad_scores[i] * LOF_i) / d points, with the kNN sets and
+ * the reach-distance max() branches fixed from a forward call on the same
+ * points, k and normalize (pass that call's nbr_idx, nbr_d2 and kdist2).
+ * Distances are recomputed from the points; sums run in FP64 in a fixed order
+ * (reverse neighbour lists, no floating-point atomics), so repeated calls are
+ * bit-identical. A zero distance contributes a zero subgradient. n_dims <= 256.
+ * No reference counterpart: the reference is forward-only. */
+int culof_lof_backward_device(const float* points, int n_points, int n_dims, int k,
+                              int normalize, const int* nbr_idx, const float* nbr_d2,
+                              const float* kdist2, const float* grad_scores, float* grad_points,
+                              void* stream);
+
 /* ---- stage entry points (device pointers), for tests and integrators ----
  * They replace the internal stage interface of the reference
  * (include/culof_internal.cuh:31-64). */
