@@ -95,7 +95,8 @@ def _check_representable(arr: NDArray[np.float32], n_features: int) -> None:
         )
 
 
-def lof(X: ArrayLike, k: int = 20, normalize: bool = False, form: str = "fp32") -> NDArray[np.float32]:
+def lof(X: ArrayLike, k: int = 20, normalize: bool = False, form: str = "fp32",
+        ties: str = "index") -> NDArray[np.float32]:
     """Local Outlier Factor of every row of ``X`` (reference: src/culof/__init__.py:83-113).
 
     Parameters
@@ -111,6 +112,11 @@ def lof(X: ArrayLike, k: int = 20, normalize: bool = False, form: str = "fp32") 
         Precision form of the distance screen. ``"bf16"`` screens candidates
         with BF16 matrix-core products and admits them on exact FP32 distances;
         it returns the same scores as ``"fp32"``. (Not in the reference.)
+    ties : {"index", "fp64"}, default="index"
+        ``"index"`` keeps the reference's rule (equal FP32 distances go to the
+        lower index). ``"fp64"`` re-selects the neighbours of rows whose k-th
+        and (k+1)-th FP32 distances are closer than FP32 resolution using FP64
+        distances, (distance, index) order. (Not in the reference.)
 
     Returns
     -------
@@ -119,12 +125,12 @@ def lof(X: ArrayLike, k: int = 20, normalize: bool = False, form: str = "fp32") 
         for one in a comparatively sparse region.
     """
     arr = _validate(X, k)
-    if form == "fp32":
+    if form == "fp32" and ties == "index":
         scores = _culof.lof(arr, k, normalize)
     else:
         from .autograd import form_code, forward_state
 
-        code = form_code(form)
+        code = form_code(form, ties)
         scores = forward_state(torch.from_numpy(arr).cuda(), k, normalize, code).scores.cpu().numpy()
     # A non-finite score would read as "not an outlier" downstream.
     if not np.isfinite(scores).all():
@@ -164,6 +170,8 @@ class LOF:
         Z-score each feature before computing distances.
     form : {"fp32", "bf16"}, default="fp32"
         Precision form of the distance screen; both give the same scores.
+    ties : {"index", "fp64"}, default="index"
+        Tie rule at the k-th neighbour; see :func:`lof`.
 
     Attributes
     ----------
@@ -181,6 +189,7 @@ class LOF:
         contamination: str | float = "auto",
         normalize: bool = False,
         form: str = "fp32",
+        ties: str = "index",
     ) -> None:
         if n_neighbors < 1:
             raise ValueError(f"n_neighbors must be >= 1, got {n_neighbors}")
@@ -190,11 +199,12 @@ class LOF:
         self.contamination = contamination
         self.normalize = normalize
         self.form = form
+        self.ties = ties
 
     def fit(self, X: ArrayLike, y: object = None) -> LOF:
         """Compute the LOF of every sample in ``X``. Returns ``self``."""
         del y  # accepted for scikit-learn call compatibility
-        scores = lof(X, self.n_neighbors, self.normalize, self.form)
+        scores = lof(X, self.n_neighbors, self.normalize, self.form, self.ties)
         self.n_neighbors_ = self.n_neighbors
         self.negative_outlier_factor_ = -scores
         if self.contamination == "auto":
@@ -235,6 +245,7 @@ class LOF:
 
     def __repr__(self) -> str:
         form = "" if self.form == "fp32" else f", form={self.form!r}"
+        form += "" if self.ties == "index" else f", ties={self.ties!r}"
         return (
             f"LOF(n_neighbors={self.n_neighbors}, "
             f"contamination={self.contamination!r}, normalize={self.normalize}{form})"

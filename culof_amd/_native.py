@@ -22,6 +22,7 @@ CULOF_NO_DEVICE = 3
 
 FORM_PLAIN = 0
 FORM_BF16 = 1
+FORM_FP64_TIES = 256  # flag OR-ed into a form
 
 # Every symbol include/culof_c.h declares, with (restype, argtypes).
 _c_int, _c_long, _c_double = ctypes.c_int, ctypes.c_long, ctypes.c_double

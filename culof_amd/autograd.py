@@ -23,17 +23,33 @@ import torch
 from . import _native
 
 FORMS = {"fp32": _native.FORM_PLAIN, "bf16": _native.FORM_BF16}
+TIES = {"index": 0, "fp64": _native.FORM_FP64_TIES}
 
 
-def form_code(form: str | int) -> int:
-    """Precision form of the distance screen: ``"fp32"`` (plain) or ``"bf16"``
-    (BF16 matrix-core screen, FP32 admission). Both give identical kNN sets."""
-    if isinstance(form, int) and form in FORMS.values():
-        return form
+def form_code(form: str | int, ties: str = "index") -> int:
+    """Form word of the C ABI.
+
+    ``form`` is the distance screen: ``"fp32"`` (plain) or ``"bf16"`` (BF16
+    matrix-core screen, FP32 admission); both give identical kNN sets.
+    ``ties`` is ``"index"`` (the reference's rule: equal FP32 keys go to the
+    lower index) or ``"fp64"`` (rows whose k/(k+1) boundary is below FP32
+    resolution are re-selected on FP64 distances, (distance, index) order).
+    An int that is already a valid form word passes through unchanged.
+    """
+    if isinstance(form, int):
+        screen = form & 0xFF
+        if form & ~(0xFF | _native.FORM_FP64_TIES) or screen not in FORMS.values():
+            raise ValueError(f"invalid form word {form:#x}")
+        code = form
+    else:
+        try:
+            code = FORMS[form]
+        except (KeyError, TypeError):
+            raise ValueError(f"form must be one of {sorted(FORMS)}, got {form!r}") from None
     try:
-        return FORMS[form]
+        return code | TIES[ties]
     except (KeyError, TypeError):
-        raise ValueError(f"form must be one of {sorted(FORMS)}, got {form!r}") from None
+        raise ValueError(f"ties must be one of {sorted(TIES)}, got {ties!r}") from None
 
 
 class Neighbours(NamedTuple):
@@ -58,7 +74,7 @@ def _check_points(X: torch.Tensor, k: int) -> torch.Tensor:
 
 
 def forward_state(X: torch.Tensor, k: int = 20, normalize: bool = False,
-                  form: str | int = "fp32") -> Neighbours:
+                  form: str | int = "fp32", ties: str = "index") -> Neighbours:
     """Run the forward on device tensor ``X`` and return scores plus the kNN state."""
     x = _check_points(X, k)
     n, d = x.shape
@@ -72,7 +88,7 @@ def forward_state(X: torch.Tensor, k: int = 20, normalize: bool = False,
     )
     with torch.cuda.device(dev):
         _native.check(_native.lib().culof_lof_device(
-            x.data_ptr(), n, d, k, int(bool(normalize)), form_code(form), out.scores.data_ptr(),
+            x.data_ptr(), n, d, k, int(bool(normalize)), form_code(form, ties), out.scores.data_ptr(),
             out.idx.data_ptr(), out.d2.data_ptr(), out.kdist2.data_ptr(), out.lrd.data_ptr(),
             _native.current_stream()))
     return out
@@ -114,10 +130,11 @@ class LOFFunction(torch.autograd.Function):
 
 
 def lof_torch(X: torch.Tensor, k: int = 20, normalize: bool = False,
-              form: str | int = "fp32") -> torch.Tensor:
+              form: str | int = "fp32", ties: str = "index") -> torch.Tensor:
     """LOF scores of the rows of device tensor ``X`` as a differentiable tensor.
 
     Same scores as :func:`culof_amd.lof` on the same float32 data; ``form``
-    picks the distance screen (``"fp32"`` or ``"bf16"``, identical results).
+    picks the distance screen (``"fp32"`` or ``"bf16"``, identical results),
+    ``ties`` the tie rule (see :func:`form_code`).
     """
-    return LOFFunction.apply(X, int(k), bool(normalize), form_code(form))
+    return LOFFunction.apply(X, int(k), bool(normalize), form_code(form, ties))

@@ -231,6 +231,25 @@ Path choose_path(int d, int k, int form, int stride) {
 // Rows a sweep hands back get the exact radix redo.
 void knn_rows(const Prepared& p, int n, int d, int k, int form, int row0, int rows, int method,
               int* nbr_idx, float* nbr_d2, float* kdist2, Scratch& scratch, hipStream_t s) {
+    if (form & kFormTiesFp64) {
+        // FP64 tie-break: select k + 1, keep rows whose boundary is clear, settle
+        // the rest in FP64. With k = n - 1 every candidate is a neighbour.
+        const int screen = form & kFormMask;
+        if (k + 1 > n - 1) {
+            knn_rows(p, n, d, k, screen, row0, rows, method, nbr_idx, nbr_d2, kdist2, scratch, s);
+            return;
+        }
+        const size_t n1 = static_cast<size_t>(n) * (k + 1);
+        int* idx1 = scratch.get<int>(n1);
+        float* d21 = scratch.get<float>(n1);
+        float* kd1 = scratch.get<float>(static_cast<size_t>(n));
+        knn_rows(p, n, d, k + 1, screen, row0, rows, method, idx1, d21, kd1, scratch, s);
+        int* tie_rows = scratch.get<int>(static_cast<size_t>(rows));
+        int* tie_count = scratch.get<int>(1);
+        tie_break_rows(p.xs, p.stride, n, d, k, row0, rows, idx1, d21, nbr_idx, nbr_d2, kdist2,
+                       tie_rows, tie_count, s);
+        return;
+    }
     const int tile = tile_rows_for(rows);
     Path path = Path::kRadix;
     if (method == 0) path = choose_path(d, k, form, p.stride);
@@ -264,7 +283,7 @@ void knn_rows(const Prepared& p, int n, int d, int k, int form, int row0, int ro
 
 void forward(const float* points, int n, int d, int k, bool normalize, int form, float* scores,
              int* nbr_idx, float* nbr_d2, float* kdist2, float* lrd, hipStream_t s) {
-    if (form != kFormPlain && form != kFormBF16) throw std::invalid_argument("unknown precision form");
+    if (screen_form(form) < 0) throw std::invalid_argument("unknown precision form");
     int device = 0;
     CULOF_CHECK(hipGetDevice(&device));
     enable_pool_reuse(device);
@@ -275,7 +294,7 @@ void forward(const float* points, int n, int d, int k, bool normalize, int form,
     if (!nbr_d2) nbr_d2 = scratch.get<float>(nk);
     if (!kdist2) kdist2 = scratch.get<float>(nn);
     if (!lrd) lrd = scratch.get<float>(nn);
-    const Prepared p = prepare(points, n, d, normalize, form, scratch, s);
+    const Prepared p = prepare(points, n, d, normalize, screen_form(form), scratch, s);
     knn_rows(p, n, d, k, form, 0, n, 0, nbr_idx, nbr_d2, kdist2, scratch, s);
     compute_lrd(nbr_idx, nbr_d2, kdist2, n, k, lrd, s);
     compute_scores(nbr_idx, lrd, n, k, scores, s);
@@ -427,14 +446,16 @@ int culof_stage_knn(const float* points, int n_points, int n_dims, int k, int no
             throw std::invalid_argument("outputs must not be null");
         if (row0 < 0 || rows < 0 || row0 + rows > n_points)
             throw std::invalid_argument("row range outside [0, n_points)");
-        if (form != kFormPlain && form != kFormBF16) throw std::invalid_argument("unknown precision form");
+        const int screen = screen_form(form);
+        if (screen < 0) throw std::invalid_argument("unknown precision form");
         if (method < 0 || method > 2) throw std::invalid_argument("method must be 0, 1 or 2");
-        if (method == 2 && (form != kFormPlain || !sweep_supported(n_dims, k)))
+        const int k_sel = (form & kFormTiesFp64) && k + 1 <= n_points - 1 ? k + 1 : k;
+        if (method == 2 && (screen != kFormPlain || !sweep_supported(n_dims, k_sel)))
             throw std::invalid_argument("no VALU sweep instantiation for this (n_dims, k, form)");
         require_device();
         const hipStream_t s = as_stream(stream);
         Scratch scratch(s);
-        const Prepared p = prepare(points, n_points, n_dims, normalize != 0, form, scratch, s);
+        const Prepared p = prepare(points, n_points, n_dims, normalize != 0, screen, scratch, s);
         knn_rows(p, n_points, n_dims, k, form, row0, rows, method, nbr_idx, nbr_d2, kdist2,
                  scratch, s);
     });

@@ -30,6 +30,15 @@ constexpr int kMaxBackwardDims = 256;
 // Precision forms of the distance screen (culof_form in culof_c.h).
 constexpr int kFormPlain = 0;  // FP32 screen
 constexpr int kFormBF16 = 1;   // BF16 MFMA screen
+constexpr int kFormMask = 0xFF;
+constexpr int kFormTiesFp64 = 0x100;  // flag: settle near-tie rows in FP64
+
+// Screen form of a form word, or -1 when the word is not a valid one.
+inline int screen_form(int form) {
+    if (form & ~(kFormMask | kFormTiesFp64)) return -1;
+    const int screen = form & kFormMask;
+    return (screen == kFormPlain || screen == kFormBF16) ? screen : -1;
+}
 
 // Feature count padded for the fused sweep: the centred rows are stored with
 // this stride, zero-filled, so a row loads as whole float4s.
@@ -99,6 +108,15 @@ void radix_knn(const float* keyrows, int stride, int n_points, int n_dims, int k
 // Stored mode: keys row-major (rows x n_cols); outputs with stride `rows`.
 void radix_select_rows(const float* keys, int rows, int n_cols, int k, int* nbr_idx,
                        float* nbr_d2, float* kdist2, hipStream_t stream);
+
+// ---- ties.hip: FP64 tie-break over a (k+1)-neighbour result ----
+// Relative band (1 + g') / (1 - g') of FP32 keys that FP64 may reorder.
+double tie_band_ratio(int d);
+// idx1 / d21: the (k+1)-lists of rows [row0, row0 + rows) (stride n). Writes
+// the k-lists of those rows; tie_rows needs `rows` ints, tie_count one int.
+void tie_break_rows(const float* keyrows, int stride, int n, int d, int k, int row0, int rows,
+                    const int* idx1, const float* d21, int* nbr_idx, float* nbr_d2, float* kdist2,
+                    int* tie_rows, int* tie_count, hipStream_t s);
 
 // ---- backward.hip: d(sum w LOF)/dX with kNN sets and max() branches fixed ----
 size_t backward_scratch_bytes(int n, int d, int k);

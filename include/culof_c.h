@@ -39,10 +39,19 @@ enum culof_status {
 
 /* Precision forms of the distance screen. Selection is always by the exact
  * FP32 difference-form keys, so both forms return the same neighbours, the
- * same k-distances and the same scores; they differ in speed only. */
+ * same k-distances and the same scores; they differ in speed only.
+ *
+ * CULOF_FORM_FP64_TIES may be OR-ed into either form: rows whose k-th and
+ * (k+1)-th FP32 keys are closer than FP32 resolution ((d + 4) 2^-24 relative,
+ * exact ties included) are re-selected on FP64 distances with the
+ * (distance, index) order of the reference's FP64 test oracle
+ * (tests/cpp/test_culof.cu:55-66); other rows are unchanged. This changes
+ * results on such rows by design: their neighbour sets become the FP64 ones.
+ * No reference counterpart. */
 enum culof_form {
-    CULOF_FORM_PLAIN = 0,      /* FP32 Gram screen (MFMA or VALU) */
-    CULOF_FORM_BF16_SCREEN = 1 /* BF16 MFMA screen, wider error margin */
+    CULOF_FORM_PLAIN = 0,        /* FP32 Gram screen (MFMA or VALU) */
+    CULOF_FORM_BF16_SCREEN = 1,  /* BF16 MFMA screen, wider error margin */
+    CULOF_FORM_FP64_TIES = 256   /* flag: FP64 tie-break of near-tie rows */
 };
 
 /* Message of the last failed call on this thread ("" if none). */

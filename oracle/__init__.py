@@ -51,6 +51,10 @@ def _load():
     lib.oracle_select_row.restype = None
     lib.oracle_knn_rows.argtypes = [_f32p, c_int, c_int, c_int, c_int, c_int, _i32p, _f32p, _f32p]
     lib.oracle_knn_rows.restype = None
+    lib.oracle_knn_ties.argtypes = [_f32p, c_int, c_int, c_int, c_int, c_int, _i32p, _f32p, _f32p, _i32p]
+    lib.oracle_knn_ties.restype = None
+    lib.oracle_tie_band_ratio.argtypes = [c_int]
+    lib.oracle_tie_band_ratio.restype = ctypes.c_double
     lib.oracle_lrd.argtypes = [_i32p, _f32p, _f32p, c_int, c_int, c_int, _f32p]
     lib.oracle_lrd.restype = None
     lib.oracle_scores.argtypes = [_i32p, _f32p, c_int, c_int, c_int, _f32p]
@@ -125,6 +129,20 @@ def knn_rows(xs, k: int, row0: int = 0, rows: int | None = None):
     kd = np.empty(rows, np.float32)
     _load().oracle_knn_rows(xs, n, d, int(k), int(row0), int(rows), idx, d2, kd)
     return idx, d2, kd
+
+
+def knn_ties(xs, k: int, row0: int = 0, rows: int | None = None):
+    """kNN with the FP64 tie-break of near-tie rows: (idx, d2, kd, tie) with
+    ``tie[r] = 1`` for rows settled on FP64 distances."""
+    xs = _as_f32(xs)
+    n, d = xs.shape
+    rows = n - row0 if rows is None else rows
+    idx = np.empty((rows, k), np.int32)
+    d2 = np.empty((rows, k), np.float32)
+    kd = np.empty(rows, np.float32)
+    tie = np.empty(rows, np.int32)
+    _load().oracle_knn_ties(xs, n, d, int(k), int(row0), int(rows), idx, d2, kd, tie)
+    return idx, d2, kd, tie
 
 
 def lrd(idx, d2, kdist2, mutation: int = 0) -> np.ndarray:

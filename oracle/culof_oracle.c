@@ -18,6 +18,8 @@
  *   oracle_scores        src/cuda/lof.cu:49-62
  *   oracle_lof           src/cuda/pipeline.cu:131-207   (the whole forward)
  *   oracle_lof_fp64      tests/cpp/test_culof.cu:39-80  (FP64 brute-force LOF)
+ *   oracle_knn_ties      the FP64 tie-break form (no reference code; its sets
+ *                        are pinned to oracle_lof_fp64's)
  *
  * Where the reference leaves an arithmetic order unspecified (cuBLAS), this
  * oracle fixes one: single-precision FMA chains in ascending feature order
@@ -230,6 +232,132 @@ void oracle_knn_rows(const float* xs, int n, int d, int k, int row0, int rows, i
     }
     free(keys);
     free(scratch);
+}
+
+/* ---------------------------------------------------------------------------
+ * FP64 tie-break of near-tie rows (an addition with no reference code; it
+ * settles rows on the (distance, index) order of the reference's FP64 test
+ * oracle, tests/cpp/test_culof.cu:55-66). With K and K' the k-th and (k+1)-th
+ * FP32 keys of a row and rho = (1 + g) / (1 - g), g = (d + 4) 2^-24: a row
+ * with K' <= K rho is re-selected over the band {j : e32_j <= K rho} by
+ * (e64, index), e64 the FP64 squared distance accumulated in ascending
+ * feature order with separate multiply and add; other rows keep the FP32
+ * result. Slot order follows the reference's rule on the keys that decided
+ * the row. `tie_out[r]` is 1 for rows settled in FP64.
+ * ------------------------------------------------------------------------- */
+
+typedef struct {
+    uint64_t e64;
+    int idx;
+    float e32;
+} band_t;
+
+static int band_cmp(const void* a, const void* b) {
+    const band_t* x = (const band_t*)a;
+    const band_t* y = (const band_t*)b;
+    if (x->e64 != y->e64) return x->e64 < y->e64 ? -1 : 1;
+    return (x->idx > y->idx) - (x->idx < y->idx);
+}
+
+static uint64_t word_of(float key, int idx) {
+    uint32_t kb = bits_of(key);
+    if (kb > 0x7F800000u) kb = 0x7F800000u;
+    return ((uint64_t)kb << 32) | (uint32_t)idx;
+}
+
+double oracle_tie_band_ratio(int d) {
+    const double g = (d + 4) * 5.9604644775390625e-8;
+    return (1.0 + g) / (1.0 - g);
+}
+
+void oracle_knn_ties(const float* xs, int n, int d, int k, int row0, int rows, int* idx_out,
+                     float* d2_out, float* kdist2_out, int* tie_out) {
+    if (k + 1 > n - 1) {
+        oracle_knn_rows(xs, n, d, k, row0, rows, idx_out, d2_out, kdist2_out);
+        for (int r = 0; r < rows; ++r) tie_out[r] = 0;
+        return;
+    }
+    const int k1 = k + 1;
+    const double rho = oracle_tie_band_ratio(d);
+    int* idx1 = (int*)malloc(sizeof(int) * (size_t)k1);
+    float* d21 = (float*)malloc(sizeof(float) * (size_t)k1);
+    band_t* band = (band_t*)malloc(sizeof(band_t) * (size_t)n);
+    for (int r = 0; r < rows; ++r) {
+        const int i = row0 + r;
+        float kd1;
+        oracle_knn_rows(xs, n, d, k1, i, 1, idx1, d21, &kd1);
+        int tmax = 0;
+        uint64_t wmax = word_of(d21[0], idx1[0]);
+        for (int t = 1; t < k1; ++t) {
+            const uint64_t w = word_of(d21[t], idx1[t]);
+            if (w > wmax) {
+                wmax = w;
+                tmax = t;
+            }
+        }
+        uint64_t wk = 0;
+        for (int t = 0; t < k1; ++t) {
+            if (t == tmax) continue;
+            const uint64_t w = word_of(d21[t], idx1[t]);
+            if (w > wk) wk = w;
+        }
+        const float K = float_of((uint32_t)(wk >> 32));
+        const float K1 = float_of((uint32_t)(wmax >> 32));
+        int* io = idx_out + (size_t)r * k;
+        float* dout = d2_out + (size_t)r * k;
+        if (isfinite(K) && (double)K1 <= (double)K * rho) {
+            tie_out[r] = 1;
+            const double bound = (double)K * rho;
+            const float* xi = xs + (size_t)i * d;
+            int m = 0;
+            for (int j = 0; j < n; ++j) {
+                if (j == i) continue;
+                const float* xj = xs + (size_t)j * d;
+                const float e = key_of(xi, xj, d);
+                if (!((double)e <= bound)) continue;
+                double s2 = 0.0;
+                for (int t = 0; t < d; ++t) {
+                    const double diff = (double)xi[t] - (double)xj[t];
+                    s2 = s2 + diff * diff;
+                }
+                uint64_t u;
+                memcpy(&u, &s2, sizeof u);
+                band[m].e64 = u;
+                band[m].idx = j;
+                band[m].e32 = e;
+                ++m;
+            }
+            qsort(band, (size_t)m, sizeof(band_t), band_cmp);
+            const uint64_t kth = band[k - 1].e64;
+            for (int t = 0; t < k; ++t) {
+                int slot = t;
+                if (band[t].e64 < kth) {
+                    slot = 0;
+                    for (int f = 0; f < k; ++f) slot += band[f].e64 < kth && band[f].idx < band[t].idx;
+                }
+                io[slot] = band[t].idx;
+                dout[slot] = band[t].e32;
+            }
+            kdist2_out[r] = band[k - 1].e32;
+        } else {
+            tie_out[r] = 0;
+            int slot = 0;
+            for (int pass = 0; pass < 2; ++pass) {
+                for (int t = 0; t < k1; ++t) {
+                    if (t == tmax) continue;
+                    if ((d21[t] < K) == (pass == 0)) {
+                        io[slot] = idx1[t];
+                        dout[slot] = d21[t];
+                        ++slot;
+                    }
+                }
+            }
+            kdist2_out[r] = K;
+        }
+    }
+    free(idx1);
+    free(d21);
+    free(band);
 }
 
 /* ---------------------------------------------------------------------------

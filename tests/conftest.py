@@ -63,6 +63,20 @@ def make_data(n: int, d: int, seed: int, n_outliers: int = 0) -> np.ndarray:
     return X
 
 
+def make_tied(n: int, d: int, seed: int, n_clusters: int = 6, grid: float = 0.05,
+              dup_frac: float = 0.02) -> np.ndarray:
+    """Clustered rows quantised to a grid (in float64, then cast) plus exact
+    duplicate rows: many equal and nearly equal FP32 distances, as in the
+    500k x 64 ties workload, at a size the oracles finish quickly."""
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(-3.0, 3.0, size=(n_clusters, d))
+    X = centres[rng.integers(0, n_clusters, size=n)] + rng.standard_normal((n, d))
+    X = np.round(X / grid) * grid
+    n_dup = int(n * dup_frac)
+    X[rng.choice(n, n_dup, replace=False)] = X[rng.integers(0, n, size=n_dup)]
+    return np.ascontiguousarray(X.astype(np.float32))
+
+
 @pytest.fixture
 def data() -> np.ndarray:
     """1,000 points in 3-D with 25 planted outliers in the first rows."""

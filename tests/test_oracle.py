@@ -10,7 +10,10 @@ Each check is chosen so that a plausible mistake in the oracle fails it:
 * the Gram-screen error bound the fused sweep relies on, on adversarial data;
 * planted mutations (the oracle's ``mutation`` argument: off-by-one
   k-distance, missing max() in the reach distance, inverted density ratio)
-  that every score check must catch.
+  that every score check must catch;
+* the FP64 tie-break restatement against the FP64 brute-force oracle's
+  neighbour sets on data full of near ties (where the plain FP32 rule is
+  shown to disagree, so the check has teeth).
 """
 
 from __future__ import annotations
@@ -21,7 +24,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import GOLDEN_CASES, load_golden, make_data, rel_err
+from conftest import GOLDEN_CASES, load_golden, make_data, make_tied, rel_err
 
 # scikit-learn's LocalOutlierFactor docstring example (sklearn/neighbors/_lof.py:175-182).
 KAT_X = np.array([[-1.1], [0.2], [101.1], [0.3]], np.float32)
@@ -217,3 +220,52 @@ def test_rejects_bad_arguments():
     for k in (0, 50):
         with pytest.raises(ValueError):
             oracle.lof(X, k)
+
+
+@pytest.mark.parametrize("n,d,k,seed", [(1500, 8, 10, 0), (1200, 3, 20, 1), (900, 32, 15, 2),
+                                        (600, 2, 40, 3)])
+def test_tie_break_matches_fp64_oracle_sets(n, d, k, seed):
+    X = make_tied(n, d, seed)
+    idx, d2, kd, tie = oracle.knn_ties(X, k)
+    pi, pd, pk = oracle.knn_rows(X, k)
+    fp64 = [set(r) for r in oracle.lof_fp64(X, k, full=True)["idx"].tolist()]
+    tied = [set(r) == f for r, f in zip(idx.tolist(), fp64)]
+    plain = [set(r) == f for r, f in zip(pi.tolist(), fp64)]
+    assert all(tied), f"{len(tied) - sum(tied)} rows differ from the FP64 oracle"
+    # rows with a clear boundary keep the FP32 result bit for bit
+    keep = tie == 0
+    np.testing.assert_array_equal(idx[keep], pi[keep])
+    np.testing.assert_array_equal(d2[keep].view(np.uint32), pd[keep].view(np.uint32))
+    np.testing.assert_array_equal(kd[keep].view(np.uint32), pk[keep].view(np.uint32))
+    # every row's slot k-1 holds a neighbour at the k-distance
+    np.testing.assert_array_equal(d2[:, -1].view(np.uint32), kd.view(np.uint32))
+    assert tie.sum() > 0
+    if seed == 0:
+        assert not all(plain), "data chosen to make the plain FP32 rule disagree"
+
+
+def test_tie_break_on_well_separated_data_changes_nothing():
+    X = make_data(800, 6, 9, 8)
+    idx, d2, kd, tie = oracle.knn_ties(X, 12)
+    pi, pd, pk = oracle.knn_rows(X, 12)
+    np.testing.assert_array_equal(idx[tie == 0], pi[tie == 0])
+    assert tie.sum() <= 2
+    fp64 = oracle.lof_fp64(X, 12, full=True)["idx"]
+    assert all(set(a) == set(b) for a, b in zip(idx.tolist(), fp64.tolist()))
+
+
+def test_tie_band_ratio_covers_fp32_rounding():
+    """The band must cover the relative error of the FP32 difference-form key."""
+    for d in (1, 8, 64, 128, 1000):
+        rho = oracle._load().oracle_tie_band_ratio(d)
+        g = (d + 2) * 2.0**-24
+        assert rho >= (1 + g) / (1 - g)
+        assert rho < 1 + 4 * (d + 4) * 2.0**-24
+    rng = np.random.default_rng(4)
+    for d in (3, 64):
+        A = rng.standard_normal((200, d)).astype(np.float32)
+        keys = np.stack([oracle.key_row(A, i) for i in range(20)])
+        exact = ((A[:20, None, :].astype(np.float64) - A[None, :, :]) ** 2).sum(-1)
+        ok = np.isfinite(keys) & (exact > 0)
+        rel = np.abs(keys[ok] - exact[ok]) / exact[ok]
+        assert rel.max() <= (d + 2) * 2.0**-24
