@@ -204,12 +204,18 @@ Prepared prepare(const float* points, int n, int d, bool normalize, int form, Sc
     return p;
 }
 
-// Which kNN path serves (d, k, form). Measured on MI355X: for narrow rows
-// (padded d <= 16) in the FP32 form the VALU-screened sweep is the faster one
-// (its 4 waves share one threshold per query); wider rows and the BF16 form go
-// to the matrix-core sweep; what neither instantiates goes to the radix path.
+// Which kNN path serves (d, k, form): the matrix-core sweep wherever it is
+// instantiated (with the candidate order and seeds it is the fastest on
+// MI355X for every measured shape), else the VALU sweep, else radix.
 // CULOF_KNN_PATH (mfma | valu | radix) pins a path, to benchmark the others.
 enum class Path { kMfma, kValu, kRadix };
+
+// CULOF_ORDER=0 streams candidates in input order with unseeded lists (the
+// results are the same either way; this is for measuring the difference).
+bool ordering_enabled() {
+    const char* env = std::getenv("CULOF_ORDER");
+    return !(env && env[0] == '0');
+}
 
 Path choose_path(int d, int k, int form, int stride) {
     const bool mfma = mfma_supported(d, k, form);
@@ -220,7 +226,6 @@ Path choose_path(int d, int k, int form, int stride) {
         if (want == "valu" && valu) return Path::kValu;
         if (want == "mfma" && mfma) return Path::kMfma;
     }
-    if (valu && padded_dims(d) <= 16) return Path::kValu;
     if (mfma) return Path::kMfma;
     if (valu) return Path::kValu;
     return Path::kRadix;
@@ -265,14 +270,28 @@ void knn_rows(const Prepared& p, int n, int d, int k, int form, int row0, int ro
     int* fb_rows = scratch.get<int>(static_cast<size_t>(rows));
     CULOF_CHECK(hipMemsetAsync(fb_count, 0, sizeof(int), s));
     void* mfma_scratch = nullptr;
+    CandidateOrder ord;
+    const int* qorder = nullptr;
+    float* tseed = nullptr;
     if (path == Path::kMfma) {
-        mfma_scratch = scratch.get<unsigned char>(mfma_scratch_bytes(n, p.stride, form, tile, k));
+        const bool order = ordering_enabled() && order_supported(p.stride);
+        mfma_scratch = scratch.get<unsigned char>(mfma_scratch_bytes(n, p.stride, form, tile, k, order));
+        if (order) {
+            // Candidates in locality order, lists seeded from nearby candidates;
+            // a full run also takes its queries in that order (coherent tiles).
+            ord = build_order(p.xs, n, p.stride,
+                              scratch.get<unsigned char>(order_scratch_bytes(n, p.stride)), s);
+            tseed = scratch.get<float>(static_cast<size_t>(n));
+            if (row0 == 0 && rows == n) qorder = ord.order;
+            seed_thresholds(p.xs, ord, qorder, row0, rows, n, k, tseed, s);
+        }
     }
     for (int r0 = 0; r0 < rows; r0 += tile) {
         const int r = std::min(tile, rows - r0);
         if (path == Path::kMfma) {
             mfma_knn(p.xs, p.xc, p.stride, p.norms, p.guard, n, d, k, form, row0 + r0, r,
-                     mfma_scratch, nbr_idx, nbr_d2, kdist2, fb_rows, fb_count, s);
+                     tseed ? &ord : nullptr, qorder ? qorder + r0 : nullptr, tseed, mfma_scratch,
+                     nbr_idx, nbr_d2, kdist2, fb_rows, fb_count, s);
         } else {
             sweep_knn(p.xs, p.xc, p.stride, p.norms, p.guard, n, d, k, row0 + r0, r, nbr_idx,
                       nbr_d2, kdist2, fb_rows, fb_count, s);

@@ -91,9 +91,15 @@ bool mfma_supported(int n_dims, int k, int form);
 int mfma_slices(int n_points, int rows);
 // Device scratch the launch needs (screen operand copy, adjusted norms,
 // per-slice partial lists).
-size_t mfma_scratch_bytes(int n_points, int dp, int form, int rows, int k);
+size_t mfma_scratch_bytes(int n_points, int dp, int form, int rows, int k, bool collect);
+struct CandidateOrder;
+// ord (optional): stream candidates in that order; qlist (optional): the
+// original indices of the `rows` queries (else row0 + local); tseed
+// (optional, indexed by original query index): an upper bound of each
+// query's k-th key from seed_thresholds.
 void mfma_knn(const float* keyrows, const float* centred, int stride, const float* norms,
               const int* guard, int n_points, int n_dims, int k, int form, int row0, int rows,
+              const CandidateOrder* ord, const int* qlist, const float* tseed,
               void* scratch, int* nbr_idx, float* nbr_d2, float* kdist2, int* fallback_rows,
               int* fallback_count, hipStream_t stream);
 
@@ -108,6 +114,21 @@ void radix_knn(const float* keyrows, int stride, int n_points, int n_dims, int k
 // Stored mode: keys row-major (rows x n_cols); outputs with stride `rows`.
 void radix_select_rows(const float* keys, int rows, int n_cols, int k, int* nbr_idx,
                        float* nbr_d2, float* kdist2, hipStream_t stream);
+
+// ---- order.hip: candidate order and per-query seeds (performance only) ----
+struct CandidateOrder {
+    int* order = nullptr;     // position -> original index
+    int* inv = nullptr;       // original index -> position
+    float* xs_ord = nullptr;  // key rows in candidate order (stride dp)
+    int dp = 0;
+};
+bool order_supported(int dp);
+size_t order_scratch_bytes(int n, int dp);
+CandidateOrder build_order(const float* xs, int n, int dp, void* scratch, hipStream_t s);
+// tseed[qi] = an upper bound of query qi's k-th key (the k-th among a window of
+// candidates near it in the order). Queries: qlist[0..rows) or [row0, row0 + rows).
+void seed_thresholds(const float* xs, const CandidateOrder& o, const int* qlist, int row0, int rows,
+                     int n, int k, float* tseed, hipStream_t s);
 
 // ---- ties.hip: FP64 tie-break over a (k+1)-neighbour result ----
 // Relative band (1 + g') / (1 - g') of FP32 keys that FP64 may reorder.

@@ -29,10 +29,10 @@
 // Screen bound. Let u = 2^-24, N the exact squared norms of the centred rows.
 // The accumulated value differs from m_i - 2 x_q.x_i by at most 2 g (N_q + N_i),
 // g = (DP + 8) u covering any FP32 accumulation order of the MFMA chain and the
-// product roundings; BF16 rounding of both operands (|d| <= 2^-8) adds
-// (2^-7 + 2^-16) (N_q + N_i). With the norm, centring and exact-key roundings
-// (about 3.2 u N and (d + 5) u T) this gives
-//   FP32: c = (2 DP + 32) u,   BF16: c = 2^-7 + 2^-14 + (2 DP + 64) u,
+// product roundings. The BF16 split form adds 3.1 2^-16 (N_q + N_i) for the
+// terms it drops (see ScreenBF16) and sums 3 DP products. With the norm,
+// centring and exact-key roundings (about 3.2 u N and (d + 5) u T) this gives
+//   FP32: c = (2 DP + 32) u,   BF16 split: c = 3.1 2^-16 + (12 DP + 64) u,
 // and the threshold arithmetic adds a 4 u (T + |x_q|^2) margin for its own
 // rounding. A true top-k candidate therefore always passes; the exact keys
 // decide, so every form selects the same neighbours.
@@ -101,6 +101,7 @@ struct ScreenF32 {
         float v[DP / 2];
     };
     using Elem = float;
+    static constexpr int kRowMul = 1;  // stored elements per feature
     // This lane's half of a row: DP / 2 contiguous floats.
     template <int DP>
     __device__ static void load_a(const Elem* half_row, Frag<DP>& f) {
@@ -136,20 +137,31 @@ struct ScreenF32 {
     __host__ __device__ static float slack(int dp) { return (2.0f * dp + 32.0f) * kU; }
 };
 
-// BF16: row i stored as RNE-rounded bf16 in natural order; instruction m
-// consumes k in [16m, 16m + 16), the lane of k-half h the 8 values at 16m + 8h.
+// BF16 split: every operand x is carried as x_hi = bf16(x) and
+// x_lo = bf16(x - x_hi) (the residual is exact in FP32), and each K step runs
+// three MFMAs, a_hi b_hi + a_hi b_lo + a_lo b_hi, accumulating in FP32. The
+// products of bf16 values are exact; the dropped a_lo b_lo term and the two
+// second roundings leave about 3 2^-16 |a_t b_t| per feature, against 2^-7
+// for plain bf16 products, at three BF16 MFMAs per step (5.3x the FP32
+// matrix rate for the same K). Row i of the operand holds DP hi values then
+// DP lo values, natural feature order; instruction m consumes k in
+// [16m, 16m + 16), the lane of k-half h the 8 values at 16m + 8h.
 struct ScreenBF16 {
     template <int DP>
     struct Frag {
-        bf16x8 v[DP / 16];
+        bf16x8 hi[DP / 16];
+        bf16x8 lo[DP / 16];
     };
     using Elem = __bf16;
-    // From this lane's first element of a row (offset 8h): 8 bf16 per MFMA.
+    static constexpr int kRowMul = 2;  // stored elements per feature
     template <int DP>
     __device__ static void load_a(const Elem* half_row, Frag<DP>& f) {
         const bf16x8* p = reinterpret_cast<const bf16x8*>(half_row);
 #pragma unroll
-        for (int m = 0; m < DP / 16; ++m) f.v[m] = p[2 * m];
+        for (int m = 0; m < DP / 16; ++m) {
+            f.hi[m] = p[2 * m];
+            f.lo[m] = p[DP / 8 + 2 * m];
+        }
     }
     __device__ static constexpr int half_offset(int, int h) { return 8 * h; }
     template <int DP>
@@ -157,17 +169,30 @@ struct ScreenBF16 {
 #pragma unroll
         for (int m = 0; m < DP / 16; ++m) {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) f.v[m][e] = static_cast<__bf16>(-2.0f * xc_row[16 * m + 8 * h + e]);
+            for (int e = 0; e < 8; ++e) {
+                const float v = -2.0f * xc_row[16 * m + 8 * h + e];
+                const __bf16 hi = static_cast<__bf16>(v);
+                f.hi[m][e] = hi;
+                f.lo[m][e] = static_cast<__bf16>(v - static_cast<float>(hi));
+            }
         }
     }
     template <int DP>
     __device__ static floatx16 run(const Frag<DP>& a, const Frag<DP>& b, floatx16 acc) {
 #pragma unroll
-        for (int m = 0; m < DP / 16; ++m) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.v[m], b.v[m], acc, 0, 0, 0);
+        for (int m = 0; m < DP / 16; ++m) {
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.lo[m], b.hi[m], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.hi[m], b.lo[m], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.hi[m], b.hi[m], acc, 0, 0, 0);
+        }
         return acc;
     }
+    // 3.1 2^-16 for the split products, FP32 accumulation of 3 DP terms whose
+    // magnitudes sum to at most 2 (N_q + N_i) (counted at 2 ulp per addition,
+    // in case the matrix core truncates), and the norm, centring and
+    // threshold roundings.
     __host__ __device__ static float slack(int dp) {
-        return 0.0078125f + 6.1035156e-5f + (2.0f * dp + 64.0f) * kU;
+        return 4.7302246e-5f + (12.0f * dp + 64.0f) * kU;
     }
 };
 
@@ -177,9 +202,8 @@ struct ScreenBF16 {
 // ds_read_b128 fall in different banks.
 template <class Op, int DP>
 struct ChunkShape {
-    static constexpr int kRowF4 = DP * static_cast<int>(sizeof(typename Op::Elem)) / 16 > 0
-                                      ? DP * static_cast<int>(sizeof(typename Op::Elem)) / 16
-                                      : 1;
+    static constexpr int kRowBytes = DP * Op::kRowMul * static_cast<int>(sizeof(typename Op::Elem));
+    static constexpr int kRowF4 = kRowBytes / 16 > 0 ? kRowBytes / 16 : 1;
     static constexpr int kPitchF4 = kRowF4 >= 4 ? kRowF4 + 1 : kRowF4;
     static constexpr int kTileBytes = 32 * kPitchF4 * 16;
     static constexpr int kFit = 16384 / kTileBytes;
@@ -205,6 +229,28 @@ __device__ __forceinline__ float exact_key(const Q& q, const float* __restrict__
         df = q[4 * v + 2] - t.z;
         e = __builtin_fmaf(df, df, e);
         df = q[4 * v + 3] - t.w;
+        e = __builtin_fmaf(df, df, e);
+    }
+    return e;
+}
+
+// exact_key with the query row in LDS (16-byte aligned), read as float4s.
+template <int DP>
+__device__ __forceinline__ float exact_key_q4(const float* q, const float* __restrict__ xs_row) {
+    const float4* q4 = reinterpret_cast<const float4*>(q);
+    const float4* c4 = reinterpret_cast<const float4*>(xs_row);
+    float e = 0.0f;
+#pragma unroll
+    for (int v = 0; v < DP / 4; ++v) {
+        const float4 a = q4[v];
+        const float4 t = c4[v];
+        float df = a.x - t.x;
+        e = __builtin_fmaf(df, df, e);
+        df = a.y - t.y;
+        e = __builtin_fmaf(df, df, e);
+        df = a.z - t.z;
+        e = __builtin_fmaf(df, df, e);
+        df = a.w - t.w;
         e = __builtin_fmaf(df, df, e);
     }
     return e;
@@ -236,11 +282,15 @@ __device__ bool emit_list(const uint64_t* stage, int k, int n, int qi, int* __re
 }
 
 struct MfmaArgs {
-    const void* xa;      // screen operand rows (form layout)
+    const void* xa;      // screen operand rows (form layout), candidate order
     const float* xc;     // centred rows, natural order, stride DP
-    const float* xs;     // key rows, stride DP
-    const float* madj;   // |x_i|^2 (1 - c)
-    const float* norms;  // |x_i|^2
+    const float* xs;     // key rows, natural order, stride DP (queries)
+    const float* xs_cand;  // key rows in candidate order (== xs without an order)
+    const int* cand;     // candidate position -> original index (null: identity)
+    const int* qlist;    // local query -> original index (null: row0 + local)
+    const float* tseed;  // per original query: upper bound of its k-th key (optional)
+    const float* madj;   // |x_i|^2 (1 - c), candidate order
+    const float* norms;  // |x_i|^2, natural order
     const int* guard;
     int n;
     int k;
@@ -249,6 +299,16 @@ struct MfmaArgs {
     int rows;
     int slices;
     uint64_t* partial;   // [slices][rows][k] when slices > 1
+    int collect;         // 1: collect kernels (needs tseed), 0: list kernels
+    const int* row_count;  // optional device count bounding rows (list kernels)
+    uint32_t* cbuf;      // collect mode: [rows][slices][2][cap] candidate positions
+    int* ccount;         // collect mode: [rows][slices][2] entries (> cap: overflowed)
+    int cap;
+    int* ov_rows;        // collect mode: rows whose slice buffer overflowed (second pass)
+    int* ov_count;
+    int ov_cap;
+    int* big_rows;       // collect mode: locals with more than kSelectMax entries
+    int* big_count;
     int* nbr_idx;
     float* nbr_d2;
     float* kdist2;
@@ -256,70 +316,18 @@ struct MfmaArgs {
     int* fb_count;
 };
 
-// Wide lists need the whole register file: let those instantiations run at
-// two waves per SIMD rather than spill the list to scratch.
-template <class Op, int DP, int KMAX>
-__global__ void __launch_bounds__(64 * kTilesPerBlock)
-__attribute__((amdgpu_waves_per_eu(1, KMAX >= 64 ? 2 : 8)))
-mfma_sweep_kernel(MfmaArgs p) {
+// Streams this block's candidate slice (blockIdx.y of p.slices) in chunks of
+// CT tiles through a double-buffered LDS stage shared by the block's 4 waves:
+// while the waves screen chunk c out of one buffer, every thread holds its
+// share of chunk c + 1 in registers and writes it to the other buffer
+// afterwards; one barrier per chunk. For each tile, visit(acc, tile) gets the
+// accumulator C = m + A B of this lane (16 candidate rows, one query). The
+// operand arrays are padded to whole tiles (zero rows, NaN norms); tiles past
+// the slice end are never visited. `stage` holds 2 * ChunkShape::kBytes.
+template <class Op, int DP, class Visit>
+__device__ __forceinline__ void stream_tiles(const MfmaArgs& p, const typename Op::template Frag<DP>& bq,
+                                             int j, int h, unsigned char* stage, Visit&& visit) {
     using Frag = typename Op::template Frag<DP>;
-    constexpr bool kQRegs = DP <= 16;  // query key row in VGPRs
-
-    // LDS: rings + two candidate chunks during the sweep; afterwards one wave
-    // at a time stages its lists in the same bytes.
-    constexpr size_t kRingBytes = sizeof(uint32_t) * kTilesPerBlock * kRing * 64;
-    constexpr size_t kChunkBytes = ChunkShape<Op, DP>::kBytes;
-    constexpr size_t kSweepBytes = kRingBytes + 2 * kChunkBytes;
-    constexpr size_t kStageBytes = sizeof(uint64_t) * KMAX * 64;
-    __shared__ __attribute__((aligned(16))) unsigned char smem[kSweepBytes > kStageBytes ? kSweepBytes : kStageBytes];
-
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int j = lane & 31;   // query within the tile
-    const int h = lane >> 5;   // k-half / candidate-row half
-    const int local = (blockIdx.x * kTilesPerBlock + wave) * 32 + j;
-    const bool valid = local < p.rows;
-    const int qi = p.row0 + (valid ? local : 0);
-    const int n = p.n;
-
-    if (*p.guard) {
-        if (valid && h == 0 && blockIdx.y == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
-        return;
-    }
-
-    // Query operands.
-    Frag bq;
-    Op::template load_b<DP>(p.xc + static_cast<size_t>(qi) * DP, h, bq);
-    float qs[kQRegs ? DP : 1];
-    if constexpr (kQRegs) {
-#pragma unroll
-        for (int t = 0; t < DP; ++t) qs[t] = p.xs[static_cast<size_t>(qi) * DP + t];
-    }
-    const float* q_global = p.xs + static_cast<size_t>(qi) * DP;
-    const float nq = p.norms[qi];
-    const float c = p.c;
-
-    uint64_t a[KMAX];
-    list_init<KMAX>(a, p.k);
-    // Each of the two lanes serving a query keeps k exact keys over its half of
-    // the candidates, so the smaller of the two k-th keys bounds the final one:
-    // both screen against it (lane l and l ^ 32 swap theirs).
-    auto threshold = [&]() {
-        float t = key_as_float(a[0]);
-        t = fminf(t, __shfl_xor(t, 32, 64));
-        if (!valid) return -__uint_as_float(kInfBits);
-        return (__builtin_fmaf(t, 1.0f + c, c * nq) - nq) + 4.0f * kU * (t + nq);
-    };
-    float Tn = threshold();
-    int cnt = 0;
-    uint32_t* ring = reinterpret_cast<uint32_t*>(smem) + wave * kRing * 64;
-
-    // Candidate slice of this block, streamed in chunks of CT tiles through a
-    // double-buffered LDS stage shared by the block's 4 waves: while the waves
-    // screen chunk c out of one buffer, every thread holds its share of chunk
-    // c + 1 in registers and writes it to the other buffer afterwards; one
-    // barrier per chunk. The operand arrays are padded to whole tiles (zero
-    // rows, NaN norms); tiles past the slice end are never screened.
     using Elem = typename Op::Elem;
     using Shape = ChunkShape<Op, DP>;
     constexpr int CT = Shape::kTiles;
@@ -332,11 +340,11 @@ mfma_sweep_kernel(MfmaArgs p) {
     constexpr int kLdsChunkF4 = CT * kLdsTileF4;
     constexpr int kLoadsA = (kChunkF4 + 255) / 256;  // per thread
     constexpr int kChunkM4 = kRowsPerChunk / 4;       // float4s of norms per chunk
-    static_assert(DP * sizeof(Elem) % 16 == 0, "screen rows must be whole float4s");
-    float4* s_a = reinterpret_cast<float4*>(smem + kRingBytes);
+    static_assert(DP * Op::kRowMul * sizeof(Elem) % 16 == 0, "screen rows must be whole float4s");
+    float4* s_a = reinterpret_cast<float4*>(stage);
     float4* s_m = s_a + 2 * kLdsChunkF4;
 
-    const int n_tiles = (n + 31) / 32;
+    const int n_tiles = (p.n + 31) / 32;
     const int t_begin = static_cast<int>((static_cast<long long>(n_tiles) * blockIdx.y) / p.slices);
     const int t_end = static_cast<int>((static_cast<long long>(n_tiles) * (blockIdx.y + 1)) / p.slices);
     const int n_chunks = (t_end - t_begin + CT - 1) / CT;
@@ -367,6 +375,122 @@ mfma_sweep_kernel(MfmaArgs p) {
         }
         if (threadIdx.x < kChunkM4) s_m[buf * kChunkM4 + threadIdx.x] = rm;
     };
+    // Accumulator initialised with the 16 adjusted norms of this lane's rows.
+    auto init_acc = [&](const float4* sm, int tt) {
+        floatx16 acc;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const float4 t = sm[tt * 8 + 2 * b + h];
+            acc[4 * b + 0] = t.x;
+            acc[4 * b + 1] = t.y;
+            acc[4 * b + 2] = t.z;
+            acc[4 * b + 3] = t.w;
+        }
+        return acc;
+    };
+
+    if (n_chunks > 0) {
+        fetch(t_begin);
+        stage_chunk(0);
+    }
+    __syncthreads();
+    for (int ch = 0; ch < n_chunks; ++ch) {
+        const int buf = ch & 1;
+        const int t0 = t_begin + ch * CT;
+        if (ch + 1 < n_chunks) fetch(t0 + CT);  // next chunk in flight
+        const float4* sa = s_a + buf * kLdsChunkF4;
+        const float4* sm = s_m + buf * kChunkM4;
+        // Two tiles per step: two independent MFMA chains hide each other's
+        // result latency.
+#pragma unroll
+        for (int tt = 0; tt < CT; tt += 2) {
+            Frag f0, f1;
+            Op::template load_a<DP>(reinterpret_cast<const Elem*>(sa + (tt * 32 + j) * kPitchF4) +
+                                        Op::half_offset(DP, h), f0);
+            floatx16 acc0 = init_acc(sm, tt);
+            const bool two = tt + 1 < CT;
+            floatx16 acc1;
+            if (two) {
+                Op::template load_a<DP>(reinterpret_cast<const Elem*>(sa + ((tt + 1) * 32 + j) * kPitchF4) +
+                                            Op::half_offset(DP, h), f1);
+                acc1 = init_acc(sm, tt + 1);
+            }
+            acc0 = Op::template run<DP>(f0, bq, acc0);
+            if (two) acc1 = Op::template run<DP>(f1, bq, acc1);
+            if (t0 + tt < t_end) visit(acc0, t0 + tt);
+            if (two && t0 + tt + 1 < t_end) visit(acc1, t0 + tt + 1);
+        }
+        if (ch + 1 < n_chunks) stage_chunk(buf ^ 1);
+        __syncthreads();
+    }
+}
+
+// Wide lists need the whole register file: let those instantiations run at
+// two waves per SIMD rather than spill the list to scratch.
+template <class Op, int DP, int KMAX>
+__global__ void __launch_bounds__(64 * kTilesPerBlock)
+__attribute__((amdgpu_waves_per_eu(1, KMAX >= 64 ? 2 : 8)))
+mfma_sweep_kernel(MfmaArgs p) {
+    using Frag = typename Op::template Frag<DP>;
+    constexpr bool kQRegs = DP <= 16;  // query key row in VGPRs
+
+    // LDS: rings + two candidate chunks during the sweep; afterwards one wave
+    // at a time stages its lists in the same bytes.
+    constexpr size_t kRingBytes = sizeof(uint32_t) * kTilesPerBlock * kRing * 64;
+    constexpr size_t kChunkBytes = ChunkShape<Op, DP>::kBytes;
+    constexpr size_t kSweepBytes = kRingBytes + 2 * kChunkBytes;
+    constexpr size_t kStageBytes = sizeof(uint64_t) * KMAX * 64;
+    __shared__ __attribute__((aligned(16))) unsigned char smem[kSweepBytes > kStageBytes ? kSweepBytes : kStageBytes];
+
+    const int rows = p.row_count ? min(p.rows, *p.row_count) : p.rows;
+    if (static_cast<int>(blockIdx.x) * kTilesPerBlock * 32 >= rows) return;  // whole block
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int j = lane & 31;   // query within the tile
+    const int h = lane >> 5;   // k-half / candidate-row half
+    const int local = (blockIdx.x * kTilesPerBlock + wave) * 32 + j;
+    const bool valid = local < rows;
+    const int qi = p.qlist ? p.qlist[valid ? local : 0] : p.row0 + (valid ? local : 0);
+    const int n = p.n;
+
+    if (*p.guard) {
+        if (valid && h == 0 && blockIdx.y == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+        return;
+    }
+
+    // Query operands.
+    Frag bq;
+    Op::template load_b<DP>(p.xc + static_cast<size_t>(qi) * DP, h, bq);
+    float qs[kQRegs ? DP : 1];
+    if constexpr (kQRegs) {
+#pragma unroll
+        for (int t = 0; t < DP; ++t) qs[t] = p.xs[static_cast<size_t>(qi) * DP + t];
+    }
+    const float* q_global = p.xs + static_cast<size_t>(qi) * DP;
+    const float nq = p.norms[qi];
+    const float c = p.c;
+
+    uint64_t a[KMAX];
+    list_init<KMAX>(a, p.k);
+    if (p.tseed) {
+        // Placeholders (T_q, 0xFFFFFFFF) admit exactly the keys <= T_q; at
+        // least k real candidates have such keys, so none survives the merge.
+        const uint64_t fake = pack(__float_as_uint(p.tseed[qi]), 0xFFFFFFFFu);
+#pragma unroll
+        for (int t = 0; t < KMAX; ++t) a[t] = (t < p.k) ? fake : 0ull;
+    }
+    // Each of the two lanes serving a query keeps k exact keys over its half of
+    // the candidates, so the smaller of the two k-th keys bounds the final one:
+    // both screen against it (lane l and l ^ 32 swap theirs).
+    auto threshold = [&]() {
+        float t = key_as_float(a[0]);
+        t = fminf(t, __shfl_xor(t, 32, 64));
+        if (!valid) return -__uint_as_float(kInfBits);
+        return (__builtin_fmaf(t, 1.0f + c, c * nq) - nq) + 4.0f * kU * (t + nq);
+    };
+    float Tn = threshold();
+    int cnt = 0;
+    uint32_t* ring = reinterpret_cast<uint32_t*>(smem) + wave * kRing * 64;
 
     auto flush = [&]() {
         // Exact keys of the queued candidates, four at a time so their row
@@ -379,7 +503,7 @@ mfma_sweep_kernel(MfmaArgs p) {
             for (int u = 0; u < 4; ++u) {
                 const bool live = r + u < cnt;
                 id[u] = live ? ring[(r + u) * 64 + lane] : 0u;
-                const float* row = p.xs + static_cast<size_t>(id[u]) * DP;
+                const float* row = p.xs_cand + static_cast<size_t>(id[u]) * DP;
                 if constexpr (kQRegs) {
                     e[u] = live ? exact_key<DP>(qs, row) : 0.0f;
                 } else {
@@ -393,9 +517,11 @@ mfma_sweep_kernel(MfmaArgs p) {
                 uint64_t w[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const uint32_t kb = (static_cast<int>(id[u]) == qi || e[u] != e[u]) ? kInfBits
+                    const bool live = r + u < cnt;
+                    const uint32_t orig = (p.cand && live) ? static_cast<uint32_t>(p.cand[id[u]]) : id[u];
+                    const uint32_t kb = (static_cast<int>(orig) == qi || e[u] != e[u]) ? kInfBits
                                                                                        : __float_as_uint(e[u]);
-                    w[u] = (r + u < cnt) ? pack(kb, id[u]) : kSentinel;
+                    w[u] = live ? pack(kb, orig) : kSentinel;
                 }
                 w0 = w[0];
                 w1 = w[1];
@@ -439,54 +565,7 @@ mfma_sweep_kernel(MfmaArgs p) {
         }
     };
 
-    // Accumulator initialised with the 16 adjusted norms of this lane's rows.
-    auto init_acc = [&](const float4* sm, int tt) {
-        floatx16 acc;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const float4 t = sm[tt * 8 + 2 * b + h];
-            acc[4 * b + 0] = t.x;
-            acc[4 * b + 1] = t.y;
-            acc[4 * b + 2] = t.z;
-            acc[4 * b + 3] = t.w;
-        }
-        return acc;
-    };
-
-    if (n_chunks > 0) {
-        fetch(t_begin);
-        stage_chunk(0);
-    }
-    __syncthreads();
-    for (int ch = 0; ch < n_chunks; ++ch) {
-        const int buf = ch & 1;
-        const int t0 = t_begin + ch * CT;
-        if (ch + 1 < n_chunks) fetch(t0 + CT);  // next chunk in flight
-        const float4* sa = s_a + buf * kLdsChunkF4;
-        const float4* sm = s_m + buf * kChunkM4;
-        // Two tiles per step: two independent MFMA chains hide each other's
-        // result latency.
-#pragma unroll
-        for (int tt = 0; tt < CT; tt += 2) {
-            Frag f0, f1;
-            Op::template load_a<DP>(reinterpret_cast<const Elem*>(sa + (tt * 32 + j) * kPitchF4) +
-                                        Op::half_offset(DP, h), f0);
-            floatx16 acc0 = init_acc(sm, tt);
-            const bool two = tt + 1 < CT;
-            floatx16 acc1;
-            if (two) {
-                Op::template load_a<DP>(reinterpret_cast<const Elem*>(sa + ((tt + 1) * 32 + j) * kPitchF4) +
-                                            Op::half_offset(DP, h), f1);
-                acc1 = init_acc(sm, tt + 1);
-            }
-            acc0 = Op::template run<DP>(f0, bq, acc0);
-            if (two) acc1 = Op::template run<DP>(f1, bq, acc1);
-            if (t0 + tt < t_end) screen(acc0, t0 + tt);
-            if (two && t0 + tt + 1 < t_end) screen(acc1, t0 + tt + 1);
-        }
-        if (ch + 1 < n_chunks) stage_chunk(buf ^ 1);
-        __syncthreads();
-    }
+    stream_tiles<Op, DP>(p, bq, j, h, smem + kRingBytes, screen);
     flush();
 
     // ---- fold the two half-lists of each query (lane j + 32 into lane j) and
@@ -530,17 +609,328 @@ mfma_sweep_kernel(MfmaArgs p) {
     }
 }
 
+// ---- collect mode: fixed per-query threshold, no list in the sweep ----
+//
+// With a seed T_q (an upper bound of the query's k-th key) the screen
+// threshold never needs to move: every candidate whose exact key can be
+// <= T_q passes, and the k nearest are among them. The sweep then only
+// appends the positions of passing candidates to a per-(query, slice) buffer
+// (the two lanes serving a query share it: in each tile the k-half 0 lane's
+// passes go first); the select kernels key them exactly and choose. Without
+// the register list the sweep runs at several waves per SIMD and spends its
+// VALU on the screen alone.
+template <class Op, int DP>
+__global__ void __launch_bounds__(64 * kTilesPerBlock)
+mfma_collect_kernel(MfmaArgs p) {
+    using Frag = typename Op::template Frag<DP>;
+    __shared__ __attribute__((aligned(16))) unsigned char smem[2 * ChunkShape<Op, DP>::kBytes];
+
+    const int rows = p.row_count ? min(p.rows, *p.row_count) : p.rows;
+    if (static_cast<int>(blockIdx.x) * kTilesPerBlock * 32 >= rows) return;  // whole block
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int j = lane & 31;
+    const int h = lane >> 5;
+    const int local = (blockIdx.x * kTilesPerBlock + wave) * 32 + j;
+    const bool valid = local < rows;
+    const int qi = p.qlist ? p.qlist[valid ? local : 0] : p.row0 + (valid ? local : 0);
+
+    if (*p.guard) {
+        if (valid && h == 0 && blockIdx.y == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+        return;
+    }
+
+    Frag bq;
+    Op::template load_b<DP>(p.xc + static_cast<size_t>(qi) * DP, h, bq);
+    const float nq = p.norms[qi];
+    const float c = p.c;
+    const float t = p.tseed[qi];
+    // Same screen inequality as the list sweep, at the fixed threshold T_q.
+    const float Tn = valid ? (__builtin_fmaf(t, 1.0f + c, c * nq) - nq) + 4.0f * kU * (t + nq)
+                           : -__uint_as_float(kInfBits);
+    const size_t slot = static_cast<size_t>(valid ? local : 0) * p.slices + blockIdx.y;
+    uint32_t* buf = p.cbuf + slot * p.cap;
+    const int cap = p.cap;
+    int cnt = 0;  // the same in both lanes of a query
+
+    auto collect = [&](const floatx16& acc, int tile) {
+        float lo = acc[0];
+#pragma unroll
+        for (int r = 1; r < 16; ++r) lo = __builtin_fminf(lo, acc[r]);  // NaN rows drop out
+        if (__any(lo <= Tn)) {
+            uint32_t mask = 0;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) mask |= (acc[r] <= Tn ? 1u : 0u) << r;
+            const int mine = __popc(mask);
+            const int other = __shfl_xor(mine, 32, 64);
+            int pos = cnt + (h ? other : 0);
+            const uint32_t c0 = static_cast<uint32_t>(tile * 32 + 4 * h);
+            while (mask) {
+                const int r = __ffs(mask) - 1;
+                mask &= mask - 1;
+                if (pos < cap) buf[pos] = c0 + static_cast<uint32_t>(8 * (r / 4) + (r % 4));
+                ++pos;
+            }
+            cnt += mine + other;
+        }
+    };
+    stream_tiles<Op, DP>(p, bq, j, h, smem, collect);
+    if (valid && h == 0) p.ccount[slot] = cnt;
+}
+
+// Collected entries per query the wave select holds; larger sets go to the
+// block select, which streams any number through LDS.
+constexpr int kSelectMax = 1024;
+constexpr int kBigChunk = 2048;  // words sorted per step of the block select
+
+__device__ __forceinline__ int lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+// (key, original index) word of candidate position `pos` for query qi.
+template <int DP>
+__device__ __forceinline__ uint64_t collected_word(const MfmaArgs& p, const float* s_q, uint32_t pos, int qi) {
+    const uint32_t orig = p.cand ? static_cast<uint32_t>(p.cand[pos]) : pos;
+    const float key = exact_key_q4<DP>(s_q, p.xs_cand + static_cast<size_t>(pos) * DP);
+    const uint32_t kb = (static_cast<int>(orig) == qi || key != key) ? kInfBits : __float_as_uint(key);
+    return pack(kb, orig);
+}
+
+// The k smallest words, given in ascending order in `sorted` (k of them), or
+// any order in `sel` with the k-th word `kth_word`: written in the reference's
+// slot order by lanes 0..k-1 of the calling wave.
+__device__ void emit_selected(const MfmaArgs& p, const uint64_t* sel, int qi, int lane) {
+    const int k = p.k;
+    uint64_t kw = 0;
+    for (int f = 0; f < k; ++f) kw = sel[f] > kw ? sel[f] : kw;
+    const uint32_t kth = static_cast<uint32_t>(kw >> 32);
+    int below = 0;
+    for (int f = 0; f < k; ++f) below += static_cast<uint32_t>(sel[f] >> 32) < kth;
+    if (lane < k) {
+        const uint64_t me = sel[lane];
+        const uint32_t key = static_cast<uint32_t>(me >> 32);
+        const uint32_t idx = static_cast<uint32_t>(me);
+        int slot = key < kth ? 0 : below;
+        for (int f = 0; f < k; ++f) {
+            const uint64_t o = sel[f];
+            const uint32_t ok = static_cast<uint32_t>(o >> 32);
+            slot += ((key < kth) ? (ok < kth) : (ok == kth)) && static_cast<uint32_t>(o) < idx;
+        }
+        const size_t ns = static_cast<size_t>(p.n);
+        p.nbr_idx[static_cast<size_t>(slot) * ns + qi] = static_cast<int>(idx);
+        p.nbr_d2[static_cast<size_t>(slot) * ns + qi] = __uint_as_float(key);
+        if (lane == 0) p.kdist2[qi] = __uint_as_float(kth);
+    }
+}
+
+// k smallest of the `total` words in s_words (WPL words per lane), emitted.
+template <int WPL>
+__device__ void select_words(const MfmaArgs& p, const uint64_t* s_words, int total, int qi, int lane,
+                             uint64_t* s_sel) {
+    uint64_t w[WPL];
+#pragma unroll
+    for (int u = 0; u < WPL; ++u) {
+        const int e = lane + 64 * u;
+        w[u] = e < total ? s_words[e] : kSentinel;
+    }
+    // ans = the largest word with fewer than k words below it: the k-th smallest.
+    uint64_t ans = 0;
+    for (int b = 63; b >= 0; --b) {
+        const uint64_t probe = ans | (1ull << b);
+        int below = 0;
+#pragma unroll
+        for (int u = 0; u < WPL; ++u) below += __popcll(__ballot(w[u] < probe));
+        if (below < p.k) ans = probe;
+    }
+    if (static_cast<uint32_t>(ans >> 32) >= kInfBits) {
+        if (lane == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+        return;
+    }
+    int off = 0;
+#pragma unroll
+    for (int u = 0; u < WPL; ++u) {
+        const bool m = w[u] <= ans;
+        const uint64_t mask = __ballot(m);
+        if (m) s_sel[off + lanes_below(mask)] = w[u];
+        off += __popcll(mask);
+    }
+    wave_lds_sync();
+    emit_selected(p, s_sel, qi, lane);
+}
+
+__device__ __forceinline__ int query_of(const MfmaArgs& p, int local) {
+    return p.qlist ? p.qlist[local] : p.row0 + local;
+}
+
+// One wave per query of the collect pass: sets of up to kSelectMax entries
+// are keyed and selected here; larger ones are queued for the block select
+// (big_rows) or, when a slice buffer overflowed, for a second collect pass
+// with larger buffers (ov_rows).
+template <int DP>
+__global__ void __launch_bounds__(256)
+collect_select_kernel(MfmaArgs p) {
+    __shared__ __attribute__((aligned(16))) float s_q[4][DP];
+    __shared__ uint32_t s_list[4][kSelectMax];
+    __shared__ uint64_t s_words[4][kSelectMax];
+    __shared__ uint64_t s_sel[4][64];
+    __shared__ int s_pre[4][33];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int local = blockIdx.x * 4 + wave;
+    if (local >= p.rows) return;  // whole waves; no block barrier below
+    if (*p.guard) return;         // the sweep queued every row for the radix path
+    const int qi = query_of(p, local);
+    const int nb = p.slices;      // <= 32
+    const size_t base = static_cast<size_t>(local) * nb;
+    const int mine = lane < nb ? p.ccount[base + lane] : 0;
+    if (lane < nb) s_pre[wave][lane + 1] = mine;
+    for (int t = lane; t < DP; t += 64) s_q[wave][t] = p.xs[static_cast<size_t>(qi) * DP + t];
+    wave_lds_sync();
+    if (lane == 0) {
+        int run = 0;
+        s_pre[wave][0] = 0;
+        for (int b = 0; b < nb; ++b) {
+            run += s_pre[wave][b + 1];
+            s_pre[wave][b + 1] = run;
+        }
+    }
+    wave_lds_sync();
+    const int total = s_pre[wave][nb];
+    if (__any(mine > p.cap)) {
+        if (lane == 0) {
+            const int at = atomicAdd(p.ov_count, 1);
+            if (at < p.ov_cap) {
+                p.ov_rows[at] = qi;
+            } else {
+                p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+            }
+        }
+        return;
+    }
+    if (total > kSelectMax) {
+        if (lane == 0) p.big_rows[atomicAdd(p.big_count, 1)] = local;
+        return;
+    }
+    if (total < p.k) {  // cannot happen with a valid seed; stay exact regardless
+        if (lane == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+        return;
+    }
+    for (int b = 0; b < nb; ++b) {
+        const int b0 = s_pre[wave][b];
+        const int cb = s_pre[wave][b + 1] - b0;
+        const uint32_t* src = p.cbuf + (base + b) * p.cap;
+        for (int i = lane; i < cb; i += 64) s_list[wave][b0 + i] = src[i];
+    }
+    wave_lds_sync();
+#pragma unroll 1
+    for (int e = lane; e < total; e += 64) s_words[wave][e] = collected_word<DP>(p, s_q[wave], s_list[wave][e], qi);
+    wave_lds_sync();
+    if (total <= 128) {
+        select_words<2>(p, s_words[wave], total, qi, lane, s_sel[wave]);
+    } else if (total <= 256) {
+        select_words<4>(p, s_words[wave], total, qi, lane, s_sel[wave]);
+    } else if (total <= 512) {
+        select_words<8>(p, s_words[wave], total, qi, lane, s_sel[wave]);
+    } else {
+        select_words<16>(p, s_words[wave], total, qi, lane, s_sel[wave]);
+    }
+}
+
+// One block per queued row (grid-stride): the row's collected entries in
+// steps of kBigChunk - k, each step sorting the new words together with the
+// best k so far (bitonic, in LDS) and keeping k. Any number of entries.
+// rows_list[r] is the row's local index in the collect pass it came from
+// (null: r itself); that pass's buffers are p.cbuf / p.ccount / p.cap.
+template <int DP>
+__global__ void __launch_bounds__(256)
+big_select_kernel(MfmaArgs p, const int* __restrict__ rows_list, const int* __restrict__ rows_count) {
+    __shared__ __attribute__((aligned(16))) float s_q[DP];
+    __shared__ uint64_t s_w[kBigChunk];
+    __shared__ int s_pre[33];
+    if (*p.guard) return;
+    const int count = min(*rows_count, p.rows);
+    const int nb = p.slices;
+    for (int r = blockIdx.x; r < count; r += gridDim.x) {
+        const int local = rows_list ? rows_list[r] : r;
+        const int qi = query_of(p, local);
+        const size_t base = static_cast<size_t>(local) * nb;
+        __syncthreads();
+        if (threadIdx.x < nb) s_pre[threadIdx.x + 1] = p.ccount[base + threadIdx.x];
+        for (int t = threadIdx.x; t < DP; t += 256) s_q[t] = p.xs[static_cast<size_t>(qi) * DP + t];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int run = 0;
+            s_pre[0] = 0;
+            bool over = false;
+            for (int b = 0; b < nb; ++b) {
+                over |= s_pre[b + 1] > p.cap;
+                run += s_pre[b + 1];
+                s_pre[b + 1] = run;
+            }
+            if (over) s_pre[0] = -1;
+        }
+        __syncthreads();
+        if (s_pre[0] < 0 || s_pre[nb] < p.k) {
+            if (threadIdx.x == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+            continue;
+        }
+        const int total = s_pre[nb];
+        int kept = 0;  // best words so far at s_w[0..kept)
+        for (int e0 = 0; e0 < total;) {
+            const int take = min(kBigChunk - kept, total - e0);
+            for (int i = threadIdx.x; i < kBigChunk - kept; i += 256) {
+                uint64_t w = kSentinel;
+                if (i < take) {
+                    const int e = e0 + i;
+                    int b = 0;
+                    while (e >= s_pre[b + 1]) ++b;
+                    const uint32_t pos = p.cbuf[(base + b) * p.cap + (e - s_pre[b])];
+                    w = collected_word<DP>(p, s_q, pos, qi);
+                }
+                s_w[kept + i] = w;
+            }
+            __syncthreads();
+            for (int size = 2; size <= kBigChunk; size <<= 1) {
+                for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                    for (int q = threadIdx.x; q < kBigChunk / 2; q += 256) {
+                        const int lo = 2 * q - (q & (stride - 1));
+                        const int hi = lo + stride;
+                        const bool up = (lo & size) == 0;
+                        const uint64_t a = s_w[lo], b2 = s_w[hi];
+                        if ((b2 < a) == up) {
+                            s_w[lo] = b2;
+                            s_w[hi] = a;
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            kept = p.k;
+            e0 += take;
+        }
+        if (threadIdx.x < 64) {
+            if (static_cast<uint32_t>(s_w[p.k - 1] >> 32) >= kInfBits) {
+                if (threadIdx.x == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+            } else {
+                emit_selected(p, s_w, qi, threadIdx.x);
+            }
+        }
+    }
+}
+
 // One thread per query: fold the slices' partial lists, then emit.
 template <int KMAX>
 __global__ void __launch_bounds__(64)
-merge_slices_kernel(const uint64_t* __restrict__ partial, int slices, int rows, int row0, int n,
-                    int k, int* __restrict__ nbr_idx, float* __restrict__ nbr_d2,
-                    float* __restrict__ kdist2, int* __restrict__ fb_rows, int* __restrict__ fb_count) {
+merge_slices_kernel(const uint64_t* __restrict__ partial, int slices, int rows, int row0,
+                    const int* __restrict__ qlist, const int* __restrict__ row_count, int n, int k, int* __restrict__ nbr_idx,
+                    float* __restrict__ nbr_d2, float* __restrict__ kdist2,
+                    int* __restrict__ fb_rows, int* __restrict__ fb_count) {
     __shared__ uint64_t stage[KMAX][64];
     const int lane = threadIdx.x;
     const int local = blockIdx.x * 64 + lane;
-    if (local >= rows) return;
-    const int qi = row0 + local;
+    if (local >= (row_count ? min(rows, *row_count) : rows)) return;
+    const int qi = qlist ? qlist[local] : row0 + local;
     uint64_t a[KMAX];
     list_init<KMAX>(a, k);
     for (int s = 0; s < slices; ++s) {
@@ -563,31 +953,54 @@ merge_slices_kernel(const uint64_t* __restrict__ partial, int slices, int rows, 
 // ---- operand preparation ----
 
 // Screen operands for rows [0, padded_rows): rows past n are zero.
+// Screen operands for candidate positions [0, padded_rows) (position p holds
+// row cand[p], or row p without an order); rows past n are zero.
 __global__ void __launch_bounds__(256)
-screen_rows_f32_kernel(const float* __restrict__ xc, int n, int padded_rows, int dp,
-                       float* __restrict__ xa) {
+screen_rows_f32_kernel(const float* __restrict__ xc, const int* __restrict__ cand, int n,
+                       int padded_rows, int dp, float* __restrict__ xa) {
     const size_t cell = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (cell >= static_cast<size_t>(padded_rows) * dp) return;
     const size_t i = cell / dp;
     const int pos = static_cast<int>(cell % dp);
     const int half = dp / 2;
     const int src = pos < half ? 2 * pos : 2 * (pos - half) + 1;
-    xa[cell] = i < static_cast<size_t>(n) ? xc[i * dp + src] : 0.0f;
+    if (i >= static_cast<size_t>(n)) {
+        xa[cell] = 0.0f;
+        return;
+    }
+    const size_t row = cand ? static_cast<size_t>(cand[i]) : i;
+    xa[cell] = xc[row * dp + src];
 }
 
+// Split BF16 operands: row p = [bf16(x) for each feature, bf16(x - bf16(x)) ...].
 __global__ void __launch_bounds__(256)
-screen_rows_bf16_kernel(const float* __restrict__ xc, size_t cells, size_t padded_cells,
-                        __bf16* __restrict__ xb) {
+screen_rows_bf16_kernel(const float* __restrict__ xc, const int* __restrict__ cand, int n,
+                        int dp, size_t padded_cells, __bf16* __restrict__ xb) {
     const size_t cell = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (cell < padded_cells) xb[cell] = static_cast<__bf16>(cell < cells ? xc[cell] : 0.0f);
+    if (cell >= padded_cells) return;
+    const size_t i = cell / dp;
+    const size_t f = cell - i * dp;
+    __bf16* row = xb + i * 2 * dp;
+    if (i >= static_cast<size_t>(n)) {
+        row[f] = static_cast<__bf16>(0.0f);
+        row[dp + f] = static_cast<__bf16>(0.0f);
+        return;
+    }
+    const size_t src = cand ? static_cast<size_t>(cand[i]) : i;
+    const float v = xc[src * dp + f];
+    const __bf16 hi = static_cast<__bf16>(v);
+    row[f] = hi;
+    row[dp + f] = static_cast<__bf16>(v - static_cast<float>(hi));
 }
 
-// |x_i|^2 (1 - c) for rows [0, padded_rows); NaN past n, which fails every compare.
+// |x|^2 (1 - c) per candidate position in [0, padded_rows); NaN past n,
+// which fails every compare.
 __global__ void __launch_bounds__(256)
-adjusted_norms_kernel(const float* __restrict__ norms, int n, int padded_rows, float c,
-                      float* __restrict__ madj) {
+adjusted_norms_kernel(const float* __restrict__ norms, const int* __restrict__ cand, int n,
+                      int padded_rows, float c, float* __restrict__ madj) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < padded_rows) madj[i] = i < n ? norms[i] * (1.0f - c) : __uint_as_float(0x7FC00000u);
+    if (i >= padded_rows) return;
+    madj[i] = i < n ? norms[cand ? cand[i] : i] * (1.0f - c) : __uint_as_float(0x7FC00000u);
 }
 
 template <class Op, int DP, int KMAX>
@@ -601,13 +1014,38 @@ void launch_mfma(const MfmaArgs& a, hipStream_t stream) {
     if (a.slices > 1) {
         ProfileScope prof("merge_slices", stream);
         merge_slices_kernel<KMAX><<<(a.rows + 63) / 64, 64, 0, stream>>>(
-            a.partial, a.slices, a.rows, a.row0, a.n, a.k, a.nbr_idx, a.nbr_d2, a.kdist2,
+            a.partial, a.slices, a.rows, a.row0, a.qlist, a.row_count, a.n, a.k, a.nbr_idx, a.nbr_d2, a.kdist2,
             a.fb_rows, a.fb_count);
     }
 }
 
 template <class Op, int DP>
+void launch_collect(const MfmaArgs& a, hipStream_t stream) {
+    const int qtiles = (a.rows + 31) / 32;
+    const dim3 grid((qtiles + kTilesPerBlock - 1) / kTilesPerBlock, a.slices);
+    const unsigned big_grid = static_cast<unsigned>(std::min(a.rows, 1024));
+    {
+        ProfileScope prof("mfma_collect", stream);
+        mfma_collect_kernel<Op, DP><<<grid, 64 * kTilesPerBlock, 0, stream>>>(a);
+    }
+    ProfileScope prof("collect_select", stream);
+    if (a.big_rows) {
+        // First pass: a wave per row, large sets to the block select.
+        collect_select_kernel<DP><<<(a.rows + 3) / 4, 256, 0, stream>>>(a);
+        big_select_kernel<DP><<<big_grid, 256, 0, stream>>>(a, a.big_rows, a.big_count);
+    } else {
+        // Second pass over overflowed rows: all large.
+        big_select_kernel<DP><<<big_grid, 256, 0, stream>>>(a, nullptr, a.row_count);
+    }
+}
+
+template <class Op, int DP>
 bool dispatch_k(const MfmaArgs* a, int k, hipStream_t stream) {
+    if (a && a->collect) {
+        if (k > 64) return false;
+        launch_collect<Op, DP>(*a, stream);
+        return true;
+    }
     if (k <= 16) {
         if (a) launch_mfma<Op, DP, 16>(*a, stream);
         return true;
@@ -670,57 +1108,126 @@ int mfma_slices(int n_points, int rows) {
     return static_cast<int>(std::max<long long>(s, 1));
 }
 
-size_t mfma_scratch_bytes(int n_points, int dp, int form, int rows, int k) {
+// Per-(query, slice, half) buffer capacity of collect mode.
+// (Each half of a slice may hold up to half of what the select keeps.)
+int collect_cap(int) { return kSelectMax / 2; }
+
+// Collect buffers: kSelectMax positions per (query, slice) in the first pass;
+// rows whose slice buffer overflows (at most kSecondRows of them) are
+// collected again with kSecondCap per slice, then go to the block select.
+// The second pass serves few rows, so it splits the candidates into many
+// short slices to keep its latency low.
+constexpr int kSecondRows = 1024;
+constexpr int kSecondCap = 4096;
+int second_slices(int n_points) {
+    const int tiles = (n_points + 31) / 32;
+    return std::max(1, std::min(32, tiles / 8));
+}
+
+size_t mfma_scratch_bytes(int n_points, int dp, int form, int rows, int k, bool collect) {
     const size_t padded = static_cast<size_t>((n_points + 31) / 32) * 32;
-    const size_t rows_bytes = form == kFormBF16 ? sizeof(uint16_t) : sizeof(float);
+    const size_t rows_bytes = form == kFormBF16 ? 2 * sizeof(uint16_t) : sizeof(float);
     const size_t operand = padded * dp * rows_bytes;
     const size_t madj = sizeof(float) * padded;
     const int s = mfma_slices(n_points, rows);
-    const size_t partial = s > 1 ? sizeof(uint64_t) * s * static_cast<size_t>(rows) * k : 0;
-    return operand + 256 + madj + 256 + partial + 256;
+    size_t lists = 0;
+    if (collect) {
+        const size_t slots = static_cast<size_t>(rows) * s;
+        const size_t second = static_cast<size_t>(std::min(rows, kSecondRows)) * second_slices(n_points);
+        lists = slots * kSelectMax * sizeof(uint32_t) + slots * sizeof(int) +
+                static_cast<size_t>(rows) * 2 * sizeof(int) + 4 * sizeof(int) +
+                second * kSecondCap * sizeof(uint32_t) + second * sizeof(int) + 8 * 256;
+    } else if (s > 1) {
+        lists = sizeof(uint64_t) * s * static_cast<size_t>(rows) * k;
+    }
+    return operand + 256 + madj + 256 + lists + 256;
 }
 
 void mfma_knn(const float* keyrows, const float* centred, int stride, const float* norms,
               const int* guard, int n_points, int n_dims, int k, int form, int row0, int rows,
+              const CandidateOrder* ord, const int* qlist, const float* tseed,
               void* scratch, int* nbr_idx, float* nbr_d2, float* kdist2, int* fallback_rows,
               int* fallback_count, hipStream_t stream) {
     const int dp = mfma_padded_dims(n_dims, form);
     if (stride != dp) throw std::invalid_argument("mfma_knn: row stride mismatch");
     if (rows <= 0) return;
+    if (ord && ord->dp != dp) throw std::invalid_argument("mfma_knn: order built for another stride");
     const bool bf16 = form == kFormBF16;
     const float c = bf16 ? ScreenBF16::slack(dp) : ScreenF32::slack(dp);
     const int padded_rows = (n_points + 31) / 32 * 32;
-    const size_t cells = static_cast<size_t>(n_points) * dp;
     const size_t padded_cells = static_cast<size_t>(padded_rows) * dp;
     unsigned char* base = static_cast<unsigned char*>(scratch);
-    void* xa = base;
-    base += (padded_cells * (bf16 ? 2 : 4) + 255) / 256 * 256;
-    float* madj = reinterpret_cast<float*>(base);
-    base += (sizeof(float) * padded_rows + 255) / 256 * 256;
-    uint64_t* partial = reinterpret_cast<uint64_t*>(base);
+    auto take = [&](size_t bytes) {
+        void* r = base;
+        base += (bytes + 255) / 256 * 256;
+        return r;
+    };
+    void* xa = take(padded_cells * 4);  // f32, or bf16 hi + lo
+    float* madj = static_cast<float*>(take(sizeof(float) * padded_rows));
+    const int slices = mfma_slices(n_points, rows);
+    if (slices > 32) throw std::invalid_argument("mfma_knn: too many slices");
+    const bool collect = tseed != nullptr;
+    MfmaArgs args{xa, centred, keyrows, ord ? ord->xs_ord : keyrows, ord ? ord->order : nullptr,
+                  qlist, tseed, madj, norms, guard, n_points, k, c, row0, rows,
+                  slices, nullptr, collect ? 1 : 0, nullptr, nullptr, nullptr, kSelectMax,
+                  nullptr, nullptr, 0, nullptr, nullptr, nbr_idx, nbr_d2, kdist2, fallback_rows,
+                  fallback_count};
+    MfmaArgs second = args;
+    if (collect) {
+        const size_t slots = static_cast<size_t>(rows) * slices;
+        args.cbuf = static_cast<uint32_t*>(take(slots * kSelectMax * sizeof(uint32_t)));
+        args.ccount = static_cast<int*>(take(slots * sizeof(int)));
+        args.big_rows = static_cast<int*>(take(static_cast<size_t>(rows) * sizeof(int)));
+        args.ov_cap = std::min(rows, kSecondRows);
+        args.ov_rows = static_cast<int*>(take(static_cast<size_t>(args.ov_cap) * sizeof(int)));
+        int* counters = static_cast<int*>(take(4 * sizeof(int)));
+        args.big_count = counters;
+        args.ov_count = counters + 1;
+        CULOF_CHECK(hipMemsetAsync(counters, 0, 4 * sizeof(int), stream));
+        // The second pass: the overflowed rows, with large buffers.
+        second = args;
+        second.qlist = args.ov_rows;
+        second.row_count = args.ov_count;
+        second.row0 = 0;
+        second.rows = args.ov_cap;
+        second.cap = kSecondCap;
+        second.slices = second_slices(n_points);
+        const size_t slots2 = static_cast<size_t>(args.ov_cap) * second.slices;
+        second.cbuf = static_cast<uint32_t*>(take(slots2 * kSecondCap * sizeof(uint32_t)));
+        second.ccount = static_cast<int*>(take(slots2 * sizeof(int)));
+        second.big_rows = nullptr;
+        second.big_count = nullptr;
+        second.ov_rows = nullptr;
+        second.ov_count = nullptr;
+        second.ov_cap = 0;
+    } else if (slices > 1) {
+        args.partial = static_cast<uint64_t*>(take(sizeof(uint64_t) * slices * static_cast<size_t>(rows) * k));
+    }
 
     {
         ProfileScope prof("screen_prep", stream);
         const unsigned grid = static_cast<unsigned>((padded_cells + 255) / 256);
+        const int* cand = ord ? ord->order : nullptr;
         if (bf16) {
-            screen_rows_bf16_kernel<<<grid, 256, 0, stream>>>(centred, cells, padded_cells,
-                                                                static_cast<__bf16*>(xa));
+            screen_rows_bf16_kernel<<<grid, 256, 0, stream>>>(centred, cand, n_points, dp,
+                                                                padded_cells, static_cast<__bf16*>(xa));
         } else {
-            screen_rows_f32_kernel<<<grid, 256, 0, stream>>>(centred, n_points, padded_rows, dp,
+            screen_rows_f32_kernel<<<grid, 256, 0, stream>>>(centred, cand, n_points, padded_rows, dp,
                                                                static_cast<float*>(xa));
         }
         CULOF_CHECK(hipGetLastError());
-        adjusted_norms_kernel<<<(padded_rows + 255) / 256, 256, 0, stream>>>(norms, n_points,
+        adjusted_norms_kernel<<<(padded_rows + 255) / 256, 256, 0, stream>>>(norms, cand, n_points,
                                                                               padded_rows, c, madj);
         CULOF_CHECK(hipGetLastError());
     }
-    const MfmaArgs args{xa, centred, keyrows, madj, norms, guard, n_points, k, c, row0, rows,
-                        mfma_slices(n_points, rows), partial, nbr_idx, nbr_d2, kdist2,
-                        fallback_rows, fallback_count};
-    const bool ok = bf16 ? dispatch<ScreenBF16>(dp, k, &args, stream)
-                         : dispatch<ScreenF32>(dp, k, &args, stream);
-    if (!ok) throw std::invalid_argument("mfma_knn: unsupported (n_dims, k, form)");
-    CULOF_CHECK(hipGetLastError());
+    auto run = [&](const MfmaArgs& a) {
+        const bool ok = bf16 ? dispatch<ScreenBF16>(dp, k, &a, stream)
+                             : dispatch<ScreenF32>(dp, k, &a, stream);
+        if (!ok) throw std::invalid_argument("mfma_knn: unsupported (n_dims, k, form)");
+        CULOF_CHECK(hipGetLastError());
+    };
+    run(args);
+    if (collect) run(second);
 }
 
 }  // namespace detail

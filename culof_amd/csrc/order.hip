@@ -1,0 +1,345 @@
+// Candidate order and per-query seeds for the matrix-core sweep.
+//
+// Neither changes any result: selection is by exact keys with the original
+// index as the tie-breaker, whatever order the candidates stream in. They
+// change how much work the sweep does. In arbitrary order each query's list
+// improves about k ln(n / k) times before it settles, and every improvement
+// is an exact re-key plus a list insertion. Here:
+//
+//   * the points are ordered by two levels of nearest-pivot cells
+//     (64 pivots, then 64 sub-pivots inside each level-1 cell), so points
+//     that are close in the order are mostly close in space;
+//   * each query gets a seed T_q: the k-th smallest exact key among a window
+//     of W candidates around its own place in that order (itself excluded).
+//     Any k candidates bound the k-th neighbour from above, so T_q >= the
+//     true k-distance; the sweep starts every list full of placeholder words
+//     (T_q, 0xFFFFFFFF), which admit exactly the candidates with key <= T_q.
+//
+// With a tight seed only candidates near the final k-th key ever pass the
+// screen, so the sweep's cost approaches that of the screen itself.
+
+#include "internal.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+
+namespace culof {
+namespace detail {
+
+namespace {
+
+constexpr int kPiv = 64;     // pivots per level
+constexpr int kBlock = 256;
+constexpr int kKeys = kPiv * kPiv;
+
+// Index of the nearest of `count` pivot rows (LDS or global) to row x, by
+// difference-form FP32 squared distance; ties to the lower pivot.
+template <int DP>
+__device__ int nearest_pivot(const float (&x)[DP], const float* piv, int count) {
+    float best = __uint_as_float(kInfBits);
+    int arg = 0;
+    for (int t = 0; t < count; ++t) {
+        const float4* p4 = reinterpret_cast<const float4*>(piv + static_cast<size_t>(t) * DP);
+        float e = 0.0f;
+#pragma unroll
+        for (int v = 0; v < DP / 4; ++v) {
+            const float4 q = p4[v];
+            float df = x[4 * v] - q.x;
+            e = __builtin_fmaf(df, df, e);
+            df = x[4 * v + 1] - q.y;
+            e = __builtin_fmaf(df, df, e);
+            df = x[4 * v + 2] - q.z;
+            e = __builtin_fmaf(df, df, e);
+            df = x[4 * v + 3] - q.w;
+            e = __builtin_fmaf(df, df, e);
+        }
+        if (e < best) {
+            best = e;
+            arg = t;
+        }
+    }
+    return arg;
+}
+
+template <int DP>
+__device__ void load_row(const float* __restrict__ row, float (&x)[DP]) {
+    const float4* r4 = reinterpret_cast<const float4*>(row);
+#pragma unroll
+    for (int v = 0; v < DP / 4; ++v) {
+        const float4 q = r4[v];
+        x[4 * v] = q.x;
+        x[4 * v + 1] = q.y;
+        x[4 * v + 2] = q.z;
+        x[4 * v + 3] = q.w;
+    }
+}
+
+// Level-1 pivots: rows floor(t n / 64) of the key rows.
+template <int DP>
+__global__ void __launch_bounds__(kBlock)
+level1_kernel(const float* __restrict__ xs, int n, int* __restrict__ cell,
+              int* __restrict__ hist) {
+    __shared__ __attribute__((aligned(16))) float piv[kPiv * DP];
+    for (int c = threadIdx.x; c < kPiv * DP; c += kBlock) {
+        const int t = c / DP;
+        const long long row = static_cast<long long>(t) * n / kPiv;
+        piv[c] = xs[row * DP + (c - t * DP)];
+    }
+    __syncthreads();
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    float x[DP];
+    load_row<DP>(xs + static_cast<size_t>(i) * DP, x);
+    const int c = nearest_pivot<DP>(x, piv, n < kPiv ? n : kPiv);
+    cell[i] = c;
+    atomicAdd(&hist[c], 1);
+}
+
+// Exclusive scan of `bins` counts (bins <= 4096) in one block; also resets
+// the cursors used by the scatter.
+__global__ void __launch_bounds__(kBlock)
+scan_bins_kernel(const int* __restrict__ hist, int bins, int* __restrict__ offset,
+                 int* __restrict__ cursor) {
+    __shared__ int part[kBlock];
+    const int per = (bins + kBlock - 1) / kBlock;
+    const int b0 = threadIdx.x * per;
+    int s = 0;
+    for (int b = b0; b < b0 + per && b < bins; ++b) s += hist[b];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int t = 0; t < kBlock; ++t) {
+            const int v = part[t];
+            part[t] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    int run = part[threadIdx.x];
+    for (int b = b0; b < b0 + per && b < bins; ++b) {
+        offset[b] = run;
+        cursor[b] = 0;
+        run += hist[b];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+scatter_kernel(const int* __restrict__ key, int n, const int* __restrict__ offset,
+               int* __restrict__ cursor, int* __restrict__ order) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int b = key[i];
+    order[offset[b] + atomicAdd(&cursor[b], 1)] = i;
+}
+
+// Level-2 pivots of every level-1 cell: members at floor(t m / 64) of the
+// cell's range in the level-1 order (fewer when the cell is smaller).
+__global__ void __launch_bounds__(kBlock)
+level2_pivots_kernel(const float* __restrict__ xs, int dp, const int* __restrict__ order1,
+                     const int* __restrict__ offset1, const int* __restrict__ hist1,
+                     float* __restrict__ piv2) {
+    const int c = blockIdx.x;
+    const int m = hist1[c];
+    for (int e = threadIdx.x; e < kPiv * dp; e += kBlock) {
+        const int t = e / dp;
+        float v = 0.0f;
+        if (t < m) {
+            const int member = order1[offset1[c] + static_cast<int>(static_cast<long long>(t) * m / kPiv)];
+            v = xs[static_cast<size_t>(member) * dp + (e - t * dp)];
+        }
+        piv2[static_cast<size_t>(c) * kPiv * dp + e] = v;
+    }
+}
+
+// Final key of each point: level-1 cell * 64 + nearest level-2 pivot of its cell.
+template <int DP>
+__global__ void __launch_bounds__(kBlock)
+level2_kernel(const float* __restrict__ xs, int n, const int* __restrict__ order1,
+              const int* __restrict__ cell1, const int* __restrict__ hist1,
+              const float* __restrict__ piv2, int* __restrict__ key, int* __restrict__ hist) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    const int i = order1[p];
+    const int c = cell1[i];
+    const int m = hist1[c];
+    float x[DP];
+    load_row<DP>(xs + static_cast<size_t>(i) * DP, x);
+    const int s = nearest_pivot<DP>(x, piv2 + static_cast<size_t>(c) * kPiv * DP, m < kPiv ? m : kPiv);
+    const int kk = c * kPiv + s;
+    key[i] = kk;
+    atomicAdd(&hist[kk], 1);
+}
+
+__global__ void __launch_bounds__(kBlock)
+inverse_kernel(const int* __restrict__ order, int n, int* __restrict__ inv) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p < n) inv[order[p]] = p;
+}
+
+// Rows of `src` (stride dp) gathered into the candidate order.
+__global__ void __launch_bounds__(kBlock)
+gather_rows_kernel(const float* __restrict__ src, int dp, const int* __restrict__ order, int n,
+                   float* __restrict__ dst) {
+    const size_t c4 = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const int per = dp / 4;
+    if (c4 >= static_cast<size_t>(n) * per) return;
+    const size_t p = c4 / per;
+    const int v = static_cast<int>(c4 - p * per);
+    reinterpret_cast<float4*>(dst)[c4] =
+        reinterpret_cast<const float4*>(src)[static_cast<size_t>(order[p]) * per + v];
+}
+
+// One wave per query: exact keys of the W candidates around the query's
+// place in the order, then their k-th smallest by a bitwise search on the
+// key bits (count of keys below a probe, summed over the wave).
+template <int DP, int W>
+__global__ void __launch_bounds__(kBlock)
+seed_kernel(const float* __restrict__ xs, const float* __restrict__ xs_ord,
+            const int* __restrict__ order, const int* __restrict__ inv, const int* __restrict__ qlist,
+            int row0, int rows, int n, int k, float* __restrict__ tseed) {
+    constexpr int kPer = W / 64;
+    const int lane = threadIdx.x & 63;
+    const int local = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (local >= rows) return;  // whole waves
+    const int qi = qlist ? qlist[local] : row0 + local;
+    const int w = n < W ? n : W;
+    int w0 = inv[qi] - w / 2;
+    w0 = w0 < 0 ? 0 : (w0 > n - w ? n - w : w0);
+    float q[DP];
+    load_row<DP>(xs + static_cast<size_t>(qi) * DP, q);
+    uint32_t key[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int c = lane + 64 * u;
+        uint32_t kb = kInfBits;
+        if (c < w) {
+            const int pos = w0 + c;
+            if (order[pos] != qi) {
+                const float4* r4 = reinterpret_cast<const float4*>(xs_ord + static_cast<size_t>(pos) * DP);
+                float e = 0.0f;
+#pragma unroll
+                for (int v = 0; v < DP / 4; ++v) {
+                    const float4 t = r4[v];
+                    float df = q[4 * v] - t.x;
+                    e = __builtin_fmaf(df, df, e);
+                    df = q[4 * v + 1] - t.y;
+                    e = __builtin_fmaf(df, df, e);
+                    df = q[4 * v + 2] - t.z;
+                    e = __builtin_fmaf(df, df, e);
+                    df = q[4 * v + 3] - t.w;
+                    e = __builtin_fmaf(df, df, e);
+                }
+                kb = (e != e) ? kInfBits : __float_as_uint(e);
+            }
+        }
+        key[u] = kb;
+    }
+    // Largest ans with #(keys < ans) < k is the k-th smallest key.
+    uint32_t ans = 0;
+    for (int b = 31; b >= 0; --b) {
+        const uint32_t probe = ans | (1u << b);
+        int below = 0;
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) below += __popcll(__ballot(key[u] < probe));
+        if (below < k) ans = probe;
+    }
+    if (lane == 0) tseed[qi] = __uint_as_float(ans > kInfBits ? kInfBits : ans);
+}
+
+template <int DP>
+void launch_order(const float* xs, int n, int* cell1, int* hist1, int* offset1, int* cursor1,
+                  int* order1, float* piv2, int* key, int* hist, int* offset, int* cursor,
+                  int* order, int* inv, hipStream_t s) {
+    const unsigned grid = static_cast<unsigned>((n + kBlock - 1) / kBlock);
+    CULOF_CHECK(hipMemsetAsync(hist1, 0, sizeof(int) * kPiv, s));
+    CULOF_CHECK(hipMemsetAsync(hist, 0, sizeof(int) * kKeys, s));
+    level1_kernel<DP><<<grid, kBlock, 0, s>>>(xs, n, cell1, hist1);
+    scan_bins_kernel<<<1, kBlock, 0, s>>>(hist1, kPiv, offset1, cursor1);
+    scatter_kernel<<<grid, kBlock, 0, s>>>(cell1, n, offset1, cursor1, order1);
+    level2_pivots_kernel<<<kPiv, kBlock, 0, s>>>(xs, DP, order1, offset1, hist1, piv2);
+    level2_kernel<DP><<<grid, kBlock, 0, s>>>(xs, n, order1, cell1, hist1, piv2, key, hist);
+    scan_bins_kernel<<<1, kBlock, 0, s>>>(hist, kKeys, offset, cursor);
+    scatter_kernel<<<grid, kBlock, 0, s>>>(key, n, offset, cursor, order);
+    inverse_kernel<<<grid, kBlock, 0, s>>>(order, n, inv);
+}
+
+template <int DP>
+void launch_seed(const float* xs, const float* xs_ord, const int* order, const int* inv,
+                 const int* qlist, int row0, int rows, int n, int k, float* tseed, hipStream_t s) {
+    const unsigned grid = static_cast<unsigned>((rows + kBlock / 64 - 1) / (kBlock / 64));
+    seed_kernel<DP, 512><<<grid, kBlock, 0, s>>>(xs, xs_ord, order, inv, qlist, row0, rows, n, k, tseed);
+}
+
+}  // namespace
+
+size_t order_scratch_bytes(int n, int dp) {
+    const size_t nn = static_cast<size_t>(n);
+    size_t b = 0;
+    b += 6 * nn * sizeof(int);                            // cell1, order1, key, order, inv, spare
+    b += static_cast<size_t>(kPiv) * kPiv * dp * sizeof(float);  // level-2 pivots
+    b += (3 * kPiv + 3 * kKeys) * sizeof(int);            // histograms, offsets, cursors
+    b += nn * dp * sizeof(float);                         // key rows in order
+    b += 16 * 256;
+    return b;
+}
+
+bool order_supported(int dp) { return dp == 4 || dp == 8 || dp == 16 || dp == 32 || dp == 64 || dp == 128; }
+
+CandidateOrder build_order(const float* xs, int n, int dp, void* scratch, hipStream_t s) {
+    if (!order_supported(dp)) throw std::invalid_argument("build_order: unsupported row stride");
+    unsigned char* p = static_cast<unsigned char*>(scratch);
+    auto take = [&](size_t bytes) {
+        void* r = p;
+        p += (bytes + 255) / 256 * 256;
+        return r;
+    };
+    const size_t nn = static_cast<size_t>(n);
+    int* cell1 = static_cast<int*>(take(nn * sizeof(int)));
+    int* order1 = static_cast<int*>(take(nn * sizeof(int)));
+    int* key = static_cast<int*>(take(nn * sizeof(int)));
+    CandidateOrder o;
+    o.order = static_cast<int*>(take(nn * sizeof(int)));
+    o.inv = static_cast<int*>(take(nn * sizeof(int)));
+    float* piv2 = static_cast<float*>(take(static_cast<size_t>(kPiv) * kPiv * dp * sizeof(float)));
+    int* hist1 = static_cast<int*>(take(kPiv * sizeof(int)));
+    int* offset1 = static_cast<int*>(take(kPiv * sizeof(int)));
+    int* cursor1 = static_cast<int*>(take(kPiv * sizeof(int)));
+    int* hist = static_cast<int*>(take(kKeys * sizeof(int)));
+    int* offset = static_cast<int*>(take(kKeys * sizeof(int)));
+    int* cursor = static_cast<int*>(take(kKeys * sizeof(int)));
+    o.xs_ord = static_cast<float*>(take(nn * dp * sizeof(float)));
+    o.dp = dp;
+    ProfileScope prof("order", s);
+    switch (dp) {
+        case 4: launch_order<4>(xs, n, cell1, hist1, offset1, cursor1, order1, piv2, key, hist, offset, cursor, o.order, o.inv, s); break;
+        case 8: launch_order<8>(xs, n, cell1, hist1, offset1, cursor1, order1, piv2, key, hist, offset, cursor, o.order, o.inv, s); break;
+        case 16: launch_order<16>(xs, n, cell1, hist1, offset1, cursor1, order1, piv2, key, hist, offset, cursor, o.order, o.inv, s); break;
+        case 32: launch_order<32>(xs, n, cell1, hist1, offset1, cursor1, order1, piv2, key, hist, offset, cursor, o.order, o.inv, s); break;
+        case 64: launch_order<64>(xs, n, cell1, hist1, offset1, cursor1, order1, piv2, key, hist, offset, cursor, o.order, o.inv, s); break;
+        default: launch_order<128>(xs, n, cell1, hist1, offset1, cursor1, order1, piv2, key, hist, offset, cursor, o.order, o.inv, s); break;
+    }
+    const size_t cells4 = nn * (dp / 4);
+    gather_rows_kernel<<<static_cast<unsigned>((cells4 + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+        xs, dp, o.order, n, o.xs_ord);
+    CULOF_CHECK(hipGetLastError());
+    return o;
+}
+
+void seed_thresholds(const float* xs, const CandidateOrder& o, const int* qlist, int row0, int rows,
+                     int n, int k, float* tseed, hipStream_t s) {
+    if (rows <= 0) return;
+    ProfileScope prof("seed", s);
+    switch (o.dp) {
+        case 4: launch_seed<4>(xs, o.xs_ord, o.order, o.inv, qlist, row0, rows, n, k, tseed, s); break;
+        case 8: launch_seed<8>(xs, o.xs_ord, o.order, o.inv, qlist, row0, rows, n, k, tseed, s); break;
+        case 16: launch_seed<16>(xs, o.xs_ord, o.order, o.inv, qlist, row0, rows, n, k, tseed, s); break;
+        case 32: launch_seed<32>(xs, o.xs_ord, o.order, o.inv, qlist, row0, rows, n, k, tseed, s); break;
+        case 64: launch_seed<64>(xs, o.xs_ord, o.order, o.inv, qlist, row0, rows, n, k, tseed, s); break;
+        default: launch_seed<128>(xs, o.xs_ord, o.order, o.inv, qlist, row0, rows, n, k, tseed, s); break;
+    }
+    CULOF_CHECK(hipGetLastError());
+}
+
+}  // namespace detail
+}  // namespace culof

@@ -205,3 +205,35 @@ def test_repeated_calls_bit_identical():
         again = forward(X, 20)
         for key in first:
             np.testing.assert_array_equal(first[key], again[key])
+
+
+def test_large_collected_sets():
+    """Rows whose candidates under the seed overflow the wave select: many
+    exact duplicates (second collect pass, block select) and distances
+    concentrated on a thin shell (block select), bit-exact with the oracle."""
+    from _gpu import knn
+
+    rng = np.random.default_rng(61)
+    dup = rng.standard_normal((6000, 5)).astype(np.float32)
+    dup[:3000] = dup[7]
+    shell = rng.standard_normal((6000, 64))
+    shell = (shell / np.linalg.norm(shell, axis=1, keepdims=True)).astype(np.float32)
+    for X, k in ((dup, 20), (shell, 32)):
+        oi, od, ok = oracle.knn_rows(X, k)
+        for form in (0, 1):
+            gi, gd, gk = knn(X, k, form=form)
+            np.testing.assert_array_equal(gi.T, oi)
+            _assert_same_bits(gd.T, od, "nbr_d2")
+            _assert_same_bits(gk, ok, "kdist2")
+
+
+def test_order_switch_gives_same_bits(monkeypatch):
+    """CULOF_ORDER=0 (input order, unseeded lists) and the default agree."""
+    from _gpu import forward
+
+    X = make_data(3000, 32, 62, 30)
+    default = forward(X, 20)
+    monkeypatch.setenv("CULOF_ORDER", "0")
+    plain = forward(X, 20)
+    for key in default:
+        np.testing.assert_array_equal(default[key], plain[key], err_msg=key)
