@@ -125,10 +125,9 @@ struct CandidateOrder {
 bool order_supported(int dp);
 size_t order_scratch_bytes(int n, int dp);
 CandidateOrder build_order(const float* xs, int n, int dp, void* scratch, hipStream_t s);
-// tseed[qi] = an upper bound of query qi's k-th key (the k-th among a window of
-// candidates near it in the order). Queries: qlist[0..rows) or [row0, row0 + rows).
-void seed_thresholds(const float* xs, const CandidateOrder& o, const int* qlist, int row0, int rows,
-                     int n, int k, float* tseed, hipStream_t s);
+// tseed[i] for every point i = an upper bound of its k-th key (the k-th among
+// a window of candidates near it in the order).
+void seed_thresholds(const CandidateOrder& o, int n, int k, float* tseed, hipStream_t s);
 
 // ---- ties.hip: FP64 tie-break over a (k+1)-neighbour result ----
 // Relative band (1 + g') / (1 - g') of FP32 keys that FP64 may reorder.

@@ -10,7 +10,8 @@
 //     (64 pivots, then 64 sub-pivots inside each level-1 cell), so points
 //     that are close in the order are mostly close in space;
 //   * each query gets a seed T_q: the k-th smallest exact key among a window
-//     of W candidates around its own place in that order (itself excluded).
+//     of W candidates around its place in that order (itself excluded; the
+//     queries of one block share the window).
 //     Any k candidates bound the k-th neighbour from above, so T_q >= the
 //     true k-distance; the sweep starts every list full of placeholder words
 //     (T_q, 0xFFFFFFFF), which admit exactly the candidates with key <= T_q.
@@ -190,61 +191,118 @@ gather_rows_kernel(const float* __restrict__ src, int dp, const int* __restrict_
         reinterpret_cast<const float4*>(src)[static_cast<size_t>(order[p]) * per + v];
 }
 
-// One wave per query: exact keys of the W candidates around the query's
-// place in the order, then their k-th smallest by a bitwise search on the
-// key bits (count of keys below a probe, summed over the wave).
-template <int DP, int W>
+// Seeds for the points at sorted positions [p0, p0 + 4 QPW) (one block, QPW
+// queries per wave): exact keys against the W candidates of a window centred
+// on the block in the candidate order, streamed through LDS in chunks of
+// 64 RPL rows (lane l keys rows l, l + 64, ... of the chunk against the
+// wave's QPW queries; the next chunk is fetched into registers meanwhile),
+// then each query's k-th smallest key by a bitwise search on the key bits.
+template <int DP, int W, int QPW, int RPL>
 __global__ void __launch_bounds__(kBlock)
-seed_kernel(const float* __restrict__ xs, const float* __restrict__ xs_ord,
-            const int* __restrict__ order, const int* __restrict__ inv, const int* __restrict__ qlist,
-            int row0, int rows, int n, int k, float* __restrict__ tseed) {
-    constexpr int kPer = W / 64;
+seed_kernel(const float* __restrict__ xs_ord, const int* __restrict__ order, int n, int k,
+            float* __restrict__ tseed) {
+    constexpr int kQ = 4 * QPW;                // queries per block
+    constexpr int kRows = 64 * RPL;            // candidate rows per chunk
+    constexpr int kChunks = W / kRows;
+    constexpr int kPitch = DP + 4;             // LDS row pitch (floats)
+    constexpr int kF4 = kRows * DP / 4;        // float4s per chunk
+    constexpr int kLoads = (kF4 + kBlock - 1) / kBlock;
+    __shared__ __attribute__((aligned(16))) float s_q[kQ][DP];
+    __shared__ __attribute__((aligned(16))) float s_c[kRows * kPitch];
     const int lane = threadIdx.x & 63;
-    const int local = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-    if (local >= rows) return;  // whole waves
-    const int qi = qlist ? qlist[local] : row0 + local;
+    const int wave = threadIdx.x >> 6;
+    const int p0 = blockIdx.x * kQ;
     const int w = n < W ? n : W;
-    int w0 = inv[qi] - w / 2;
-    w0 = w0 < 0 ? 0 : (w0 > n - w ? n - w : w0);
-    float q[DP];
-    load_row<DP>(xs + static_cast<size_t>(qi) * DP, q);
-    uint32_t key[kPer];
+    int c0 = p0 + kQ / 2 - w / 2;
+    c0 = c0 < 0 ? 0 : (c0 > n - w ? n - w : c0);
+    for (int e = threadIdx.x; e < kQ * DP / 4; e += kBlock) {
+        const int t = e / (DP / 4);
+        const int v = e - t * (DP / 4);
+        const int pos = p0 + t < n ? p0 + t : n - 1;
+        reinterpret_cast<float4*>(&s_q[t][0])[v] =
+            reinterpret_cast<const float4*>(xs_ord + static_cast<size_t>(pos) * DP)[v];
+    }
+    float4 nxt[kLoads];
+    auto fetch = [&](int ch) {
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-        const int c = lane + 64 * u;
-        uint32_t kb = kInfBits;
-        if (c < w) {
-            const int pos = w0 + c;
-            if (order[pos] != qi) {
-                const float4* r4 = reinterpret_cast<const float4*>(xs_ord + static_cast<size_t>(pos) * DP);
-                float e = 0.0f;
+        for (int u = 0; u < kLoads; ++u) {
+            const int e = threadIdx.x + kBlock * u;
+            const int r = e / (DP / 4);
+            const int c = ch * kRows + r;
+            nxt[u] = (e < kF4 && c < w)
+                         ? reinterpret_cast<const float4*>(xs_ord + static_cast<size_t>(c0 + c) * DP)[e - r * (DP / 4)]
+                         : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    };
+    fetch(0);
+    // The keys of each query, W / 64 per lane, kept in a shift register so the
+    // chunk loop can stay rolled: every chunk shifts RPL old keys out of the
+    // front and its RPL new keys in at the back.
+    uint32_t key[QPW][W / 64];
+#pragma unroll 1
+    for (int ch = 0; ch < kChunks; ++ch) {
+        __syncthreads();
 #pragma unroll
-                for (int v = 0; v < DP / 4; ++v) {
-                    const float4 t = r4[v];
-                    float df = q[4 * v] - t.x;
-                    e = __builtin_fmaf(df, df, e);
-                    df = q[4 * v + 1] - t.y;
-                    e = __builtin_fmaf(df, df, e);
-                    df = q[4 * v + 2] - t.z;
-                    e = __builtin_fmaf(df, df, e);
-                    df = q[4 * v + 3] - t.w;
-                    e = __builtin_fmaf(df, df, e);
+        for (int u = 0; u < kLoads; ++u) {
+            const int e = threadIdx.x + kBlock * u;
+            const int r = e / (DP / 4);
+            if (e < kF4) reinterpret_cast<float4*>(s_c + r * kPitch)[e - r * (DP / 4)] = nxt[u];
+        }
+        __syncthreads();
+        if (ch + 1 < kChunks) fetch(ch + 1);
+        uint32_t fresh[QPW][RPL];
+#pragma unroll
+        for (int rr = 0; rr < RPL; ++rr) {
+            const int r = lane + 64 * rr;
+            float e[QPW];
+#pragma unroll
+            for (int t = 0; t < QPW; ++t) e[t] = 0.0f;
+            const float4* row = reinterpret_cast<const float4*>(s_c + r * kPitch);
+#pragma unroll 4
+            for (int v = 0; v < DP / 4; ++v) {
+                const float4 cv = row[v];
+#pragma unroll
+                for (int t = 0; t < QPW; ++t) {
+                    const float4 qv = reinterpret_cast<const float4*>(&s_q[wave * QPW + t][0])[v];
+                    float df = qv.x - cv.x;
+                    e[t] = __builtin_fmaf(df, df, e[t]);
+                    df = qv.y - cv.y;
+                    e[t] = __builtin_fmaf(df, df, e[t]);
+                    df = qv.z - cv.z;
+                    e[t] = __builtin_fmaf(df, df, e[t]);
+                    df = qv.w - cv.w;
+                    e[t] = __builtin_fmaf(df, df, e[t]);
                 }
-                kb = (e != e) ? kInfBits : __float_as_uint(e);
+            }
+            const int c = ch * kRows + r;
+#pragma unroll
+            for (int t = 0; t < QPW; ++t) {
+                const bool self = c0 + c == p0 + wave * QPW + t;
+                fresh[t][rr] = (c >= w || self || e[t] != e[t]) ? kInfBits : __float_as_uint(e[t]);
             }
         }
-        key[u] = kb;
+#pragma unroll
+        for (int t = 0; t < QPW; ++t) {
+#pragma unroll
+            for (int u = 0; u + RPL < W / 64; ++u) key[t][u] = key[t][u + RPL];
+#pragma unroll
+            for (int rr = 0; rr < RPL; ++rr) key[t][W / 64 - RPL + rr] = fresh[t][rr];
+        }
     }
     // Largest ans with #(keys < ans) < k is the k-th smallest key.
-    uint32_t ans = 0;
-    for (int b = 31; b >= 0; --b) {
-        const uint32_t probe = ans | (1u << b);
-        int below = 0;
 #pragma unroll
-        for (int u = 0; u < kPer; ++u) below += __popcll(__ballot(key[u] < probe));
-        if (below < k) ans = probe;
+    for (int t = 0; t < QPW; ++t) {
+        const int pos = p0 + wave * QPW + t;
+        uint32_t ans = 0;
+        for (int b = 31; b >= 0; --b) {
+            const uint32_t probe = ans | (1u << b);
+            int below = 0;
+#pragma unroll
+            for (int u = 0; u < W / 64; ++u) below += __popcll(__ballot(key[t][u] < probe));
+            if (below < k) ans = probe;
+        }
+        if (lane == 0 && pos < n) tseed[order[pos]] = __uint_as_float(ans > kInfBits ? kInfBits : ans);
     }
-    if (lane == 0) tseed[qi] = __uint_as_float(ans > kInfBits ? kInfBits : ans);
 }
 
 template <int DP>
@@ -265,10 +323,13 @@ void launch_order(const float* xs, int n, int* cell1, int* hist1, int* offset1, 
 }
 
 template <int DP>
-void launch_seed(const float* xs, const float* xs_ord, const int* order, const int* inv,
-                 const int* qlist, int row0, int rows, int n, int k, float* tseed, hipStream_t s) {
-    const unsigned grid = static_cast<unsigned>((rows + kBlock / 64 - 1) / (kBlock / 64));
-    seed_kernel<DP, 512><<<grid, kBlock, 0, s>>>(xs, xs_ord, order, inv, qlist, row0, rows, n, k, tseed);
+void launch_seed(const float* xs_ord, const int* order, int n, int k, float* tseed, hipStream_t s) {
+    // Narrow rows afford a wider window (tighter seeds) for the same work.
+    constexpr int W = (DP <= 8 || DP > 32) ? 1024 : 2048;
+    constexpr int QPW = DP <= 32 ? 2 : 4;
+    constexpr int RPL = DP <= 8 ? 8 : (DP <= 32 ? 2 : 1);  // about 16 KiB of rows per chunk
+    const unsigned grid = static_cast<unsigned>((n + 4 * QPW - 1) / (4 * QPW));
+    seed_kernel<DP, W, QPW, RPL><<<grid, kBlock, 0, s>>>(xs_ord, order, n, k, tseed);
 }
 
 }  // namespace
@@ -326,17 +387,15 @@ CandidateOrder build_order(const float* xs, int n, int dp, void* scratch, hipStr
     return o;
 }
 
-void seed_thresholds(const float* xs, const CandidateOrder& o, const int* qlist, int row0, int rows,
-                     int n, int k, float* tseed, hipStream_t s) {
-    if (rows <= 0) return;
+void seed_thresholds(const CandidateOrder& o, int n, int k, float* tseed, hipStream_t s) {
     ProfileScope prof("seed", s);
     switch (o.dp) {
-        case 4: launch_seed<4>(xs, o.xs_ord, o.order, o.inv, qlist, row0, rows, n, k, tseed, s); break;
-        case 8: launch_seed<8>(xs, o.xs_ord, o.order, o.inv, qlist, row0, rows, n, k, tseed, s); break;
-        case 16: launch_seed<16>(xs, o.xs_ord, o.order, o.inv, qlist, row0, rows, n, k, tseed, s); break;
-        case 32: launch_seed<32>(xs, o.xs_ord, o.order, o.inv, qlist, row0, rows, n, k, tseed, s); break;
-        case 64: launch_seed<64>(xs, o.xs_ord, o.order, o.inv, qlist, row0, rows, n, k, tseed, s); break;
-        default: launch_seed<128>(xs, o.xs_ord, o.order, o.inv, qlist, row0, rows, n, k, tseed, s); break;
+        case 4: launch_seed<4>(o.xs_ord, o.order, n, k, tseed, s); break;
+        case 8: launch_seed<8>(o.xs_ord, o.order, n, k, tseed, s); break;
+        case 16: launch_seed<16>(o.xs_ord, o.order, n, k, tseed, s); break;
+        case 32: launch_seed<32>(o.xs_ord, o.order, n, k, tseed, s); break;
+        case 64: launch_seed<64>(o.xs_ord, o.order, n, k, tseed, s); break;
+        default: launch_seed<128>(o.xs_ord, o.order, n, k, tseed, s); break;
     }
     CULOF_CHECK(hipGetLastError());
 }
