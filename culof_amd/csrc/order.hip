@@ -86,14 +86,20 @@ level1_kernel(const float* __restrict__ xs, int n, int* __restrict__ cell,
         const long long row = static_cast<long long>(t) * n / kPiv;
         piv[c] = xs[row * DP + (c - t * DP)];
     }
+    __shared__ int s_hist[kPiv];
+    if (threadIdx.x < kPiv) s_hist[threadIdx.x] = 0;
     __syncthreads();
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    float x[DP];
-    load_row<DP>(xs + static_cast<size_t>(i) * DP, x);
-    const int c = nearest_pivot<DP>(x, piv, n < kPiv ? n : kPiv);
-    cell[i] = c;
-    atomicAdd(&hist[c], 1);
+    if (i < n) {
+        float x[DP];
+        load_row<DP>(xs + static_cast<size_t>(i) * DP, x);
+        const int c = nearest_pivot<DP>(x, piv, n < kPiv ? n : kPiv);
+        cell[i] = c;
+        atomicAdd(&s_hist[c], 1);
+    }
+    __syncthreads();
+    // One global add per bin per block: the 64 bins are hot.
+    if (threadIdx.x < kPiv && s_hist[threadIdx.x]) atomicAdd(&hist[threadIdx.x], s_hist[threadIdx.x]);
 }
 
 // Exclusive scan of `bins` counts (bins <= 4096) in one block; also resets
@@ -132,6 +138,25 @@ scatter_kernel(const int* __restrict__ key, int n, const int* __restrict__ offse
     if (i >= n) return;
     const int b = key[i];
     order[offset[b] + atomicAdd(&cursor[b], 1)] = i;
+}
+
+// Scatter for few hot bins (<= kPiv): ranks within the block from LDS
+// atomics, then one global reservation per bin per block.
+__global__ void __launch_bounds__(kBlock)
+scatter_few_kernel(const int* __restrict__ key, int n, const int* __restrict__ offset,
+                   int* __restrict__ cursor, int* __restrict__ order) {
+    __shared__ int s_cnt[kPiv];
+    __shared__ int s_base[kPiv];
+    if (threadIdx.x < kPiv) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int b = i < n ? key[i] : 0;
+    const int rank = i < n ? atomicAdd(&s_cnt[b], 1) : 0;
+    __syncthreads();
+    if (threadIdx.x < kPiv && s_cnt[threadIdx.x])
+        s_base[threadIdx.x] = offset[threadIdx.x] + atomicAdd(&cursor[threadIdx.x], s_cnt[threadIdx.x]);
+    __syncthreads();
+    if (i < n) order[s_base[b] + rank] = i;
 }
 
 // Level-2 pivots of every level-1 cell: members at floor(t m / 64) of the
@@ -314,7 +339,7 @@ void launch_order(const float* xs, int n, int* cell1, int* hist1, int* offset1, 
     CULOF_CHECK(hipMemsetAsync(hist, 0, sizeof(int) * kKeys, s));
     level1_kernel<DP><<<grid, kBlock, 0, s>>>(xs, n, cell1, hist1);
     scan_bins_kernel<<<1, kBlock, 0, s>>>(hist1, kPiv, offset1, cursor1);
-    scatter_kernel<<<grid, kBlock, 0, s>>>(cell1, n, offset1, cursor1, order1);
+    scatter_few_kernel<<<grid, kBlock, 0, s>>>(cell1, n, offset1, cursor1, order1);
     level2_pivots_kernel<<<kPiv, kBlock, 0, s>>>(xs, DP, order1, offset1, hist1, piv2);
     level2_kernel<DP><<<grid, kBlock, 0, s>>>(xs, n, order1, cell1, hist1, piv2, key, hist);
     scan_bins_kernel<<<1, kBlock, 0, s>>>(hist, kKeys, offset, cursor);
