@@ -24,7 +24,8 @@ torch.cuda.synchronize(). Prints one JSON line:
                k=20 (points/s of both passes, and the ratio to the forward)
 * forms        the BF16-screen form against the plain FP32 form at n=100k,
                d=128, k=50: points/s of each and whether the kNN sets, the
-               k-distances and the scores are identical
+               k-distances and the scores are identical (and both forms on
+               the headline workload)
 """
 
 from __future__ import annotations
@@ -46,8 +47,9 @@ FP32_PEAK_TFLOPS = 157.3      # MI355X dense FP32 (matrix and packed vector)
 FP32_VALU_TFLOPS = 78.6       # non-packed FP32 VALU (packed FP32 ops are disabled in this build)
 HBM_PEAK_GBS = 8000.0
 REFERENCE_PTS_S = 1.42e5      # reference on RTX A6000 at n=200k, d=8, k=20 (its README.md:50)
-KERNELS = ("center_partials", "center_apply", "screen_prep", "mfma_sweep", "merge_slices",
-           "sweep_knn", "radix_knn", "lrd", "score")
+KERNELS = ("center_partials", "center_apply", "order", "seed", "screen_prep", "mfma_collect",
+           "second_collect", "collect_select", "mfma_sweep", "merge_slices", "sweep_knn",
+           "radix_knn", "tie_break", "lrd", "score")
 
 
 def log(msg: str) -> None:
@@ -86,9 +88,9 @@ def calibrate_copy(mib: int) -> int:
     return 0
 
 
-def read_traffic(workload: str):
-    """HBM bytes per sweep launch from the committed rocprofv3 counter summary."""
-    path = os.path.join(ROOT, "profiles", f"knn_sweep_{workload}_counters.md")
+def read_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_counters.md")
     if not os.path.exists(path):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
@@ -165,6 +167,24 @@ def forms_legs(seed: int, steps: int) -> dict:
         "identical_knn": same_idx and same_kd, "identical_scores": same_scores,
     }
     log(f"[bench] forms {out['forms']}")
+    del x, res
+
+    # The headline workload in both screen forms (the headline value uses the plain form).
+    X = workloads.make("plain-200k", seed=seed)
+    n, d = X.shape
+    k = workloads.WORKLOADS["plain-200k"]["k"]
+    x = torch.from_numpy(X).cuda()
+    res = {}
+    for form in ("fp32", "bf16"):
+        t = _time_device(lambda f=form: res.__setitem__(f, forward_state(x, k, form=f)), steps)
+        res[form + "_s"] = t
+    out["forms_headline"] = {
+        "workload": "plain-200k", "n": n, "d": d, "k": k,
+        "fp32_points_per_s": round(n / res["fp32_s"], 2),
+        "bf16_points_per_s": round(n / res["bf16_s"], 2),
+        "identical_scores": bool(torch.equal(res["fp32"].scores, res["bf16"].scores)),
+    }
+    log(f"[bench] forms (headline workload) {out['forms_headline']}")
     return out
 
 
@@ -242,15 +262,15 @@ def main() -> int:
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-        "traffic": read_traffic(args.workload),
+        "traffic": read_traffic(dominant, args.workload),
         "kernel": dominant,
         "avg_launch_ms": round(avg_sweep_s * 1e3, 4),
         "frac_of_valu_fp32_peak": round(achieved / FP32_VALU_TFLOPS, 4),
         "select_bytes_effective_gbs": round(20.0 * pairs / avg_sweep_s / 1e9, 1),
-        "note": "achieved counts the 2*d flops per query-candidate pair of the distance "
-                "screen over every pair of the forward; selection never stores distances, "
-                "so select_bytes_effective_gbs (20 B/pair, the stored-tile figure) is an "
-                "equivalent, not HBM traffic",
+        "note": "achieved counts the 2*d flops per query-candidate pair of the matrix-core "
+                "distance screen (first collect pass, FP32 MFMA) over every pair of the "
+                "forward; selection never stores distances, so select_bytes_effective_gbs "
+                "(20 B/pair, the stored-tile figure) is an equivalent, not HBM traffic",
     }
     kernels = {name: {"ms_per_step": round(ms / args.steps, 4), "launches": cnt}
                for name, (ms, cnt) in total_kernel.items() if cnt}

@@ -1025,7 +1025,7 @@ void launch_collect(const MfmaArgs& a, hipStream_t stream) {
     const dim3 grid((qtiles + kTilesPerBlock - 1) / kTilesPerBlock, a.slices);
     const unsigned big_grid = static_cast<unsigned>(std::min(a.rows, 1024));
     {
-        ProfileScope prof("mfma_collect", stream);
+        ProfileScope prof(a.big_rows ? "mfma_collect" : "second_collect", stream);
         mfma_collect_kernel<Op, DP><<<grid, 64 * kTilesPerBlock, 0, stream>>>(a);
     }
     ProfileScope prof("collect_select", stream);
