@@ -269,3 +269,55 @@ def test_tie_band_ratio_covers_fp32_rounding():
         ok = np.isfinite(keys) & (exact > 0)
         rel = np.abs(keys[ok] - exact[ok]) / exact[ok]
         assert rel.max() <= (d + 2) * 2.0**-24
+
+
+def _bf16(v):
+    """Round-to-nearest-even float32 -> bfloat16, returned as float32."""
+    b = np.ascontiguousarray(v, np.float32).view(np.uint32).astype(np.uint64)
+    b = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
+    return b.astype(np.uint32).view(np.float32)
+
+
+@pytest.mark.parametrize("name", ["far", "clusters", "duplicates", "wide", "boundary"])
+def test_split_bf16_screen_bound_holds(name):
+    """The split BF16 screen (a_hi b_hi + a_hi b_lo + a_lo b_hi, bf16 products
+    exact) drops at most 3.1 2^-16 sum_t |a_t b_t|, and with FP32 accumulation
+    of the 3 d products (any order) stays within c (|a|^2 + |b|^2),
+    c = 3.1 2^-16 + (12 d + 64) 2^-24, for b = -2 q."""
+    sets = dict(_adversarial_sets())
+    rng = np.random.default_rng(5)
+    # values just off the bf16 rounding midpoints, magnitudes spread over 2^40
+    mant = (1.0 + (rng.integers(0, 128, size=(64, 48)) + 0.5 + rng.uniform(-1e-3, 1e-3, (64, 48))) / 128)
+    sets["boundary"] = (mant * 2.0 ** rng.integers(-20, 20, size=(64, 48)) *
+                        rng.choice([-1.0, 1.0], size=(64, 48))).astype(np.float32)
+    X = sets[name]
+    xc, norms, _, _ = oracle.center(X) if name != "boundary" else (X, (X.astype(np.float64) ** 2).sum(1), 0, 0)
+    xc = np.asarray(xc, np.float32)
+    d = xc.shape[1]
+    hi = _bf16(xc)
+    lo = _bf16((xc - hi).astype(np.float32))
+    c = 3.1 * 2.0 ** -16 + (12 * d + 64) * 2.0 ** -24
+    worst_drop = worst_total = 0.0
+    for i in range(0, xc.shape[0], max(1, xc.shape[0] // 24)):
+        b = (-2.0 * xc[i]).astype(np.float32)          # exact scaling
+        bh = _bf16(b)
+        bl = _bf16((b - bh).astype(np.float32))
+        exact = xc.astype(np.float64) @ b.astype(np.float64)
+        prods = np.concatenate([hi.astype(np.float64) * bh, hi.astype(np.float64) * bl,
+                                lo.astype(np.float64) * bh], axis=1)   # exact in FP64
+        drop = np.abs(prods.sum(1) - exact)
+        mag = np.abs(xc.astype(np.float64)) @ np.abs(b.astype(np.float64))
+        ok = mag > 0
+        worst_drop = max(worst_drop, float((drop[ok] / (3.1 * 2.0 ** -16 * mag[ok])).max(initial=0)))
+        # FP32 accumulation in two different orders
+        nn = np.asarray(norms, np.float64)
+        scale = nn + float(nn[i]) if name != "boundary" else nn + nn[i]
+        for order in (np.arange(prods.shape[1]), rng.permutation(prods.shape[1])):
+            acc = np.zeros(xc.shape[0], np.float32)
+            for t in order:
+                acc = (acc + prods[:, t].astype(np.float32)).astype(np.float32)
+            err = np.abs(acc.astype(np.float64) - exact)
+            sel = scale > 0
+            worst_total = max(worst_total, float((err[sel] / (c * scale[sel])).max(initial=0)))
+    assert worst_drop <= 1.0, worst_drop
+    assert worst_total <= 1.0, worst_total
