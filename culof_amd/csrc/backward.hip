@@ -156,13 +156,7 @@ __device__ void sift_down(int* a, int start, int end) {
     }
 }
 
-__global__ void __launch_bounds__(kBlock)
-rev_sort_kernel(int* __restrict__ rev, const int* __restrict__ offset,
-                const int* __restrict__ count, int n) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    int* a = rev + offset[p];
-    const int len = count[p];
+__device__ void heap_sort(int* a, int len) {
     if (len < 2) return;
     for (int start = (len - 2) / 2; start >= 0; --start) sift_down(a, start, len - 1);
     for (int end = len - 1; end > 0; --end) {
@@ -171,6 +165,47 @@ rev_sort_kernel(int* __restrict__ rev, const int* __restrict__ offset,
         a[end] = t;
         sift_down(a, 0, end - 1);
     }
+}
+
+// The kBlock consecutive points of a block own one contiguous CSR range; when
+// it fits (kSortLds entries) it is sorted in LDS, each thread its own segment
+// (insertion sort for short ones), else each segment in place in global memory.
+constexpr int kSortLds = 8192;
+
+__global__ void __launch_bounds__(kBlock)
+rev_sort_kernel(int* __restrict__ rev, const int* __restrict__ offset,
+                const int* __restrict__ count, int n) {
+    __shared__ int s_seg[kSortLds];
+    const int p0 = blockIdx.x * kBlock;
+    const int p_last = min(p0 + kBlock, n) - 1;
+    const int begin = offset[p0];
+    const int span = offset[p_last] + count[p_last] - begin;
+    const int p = p0 + threadIdx.x;
+    if (span > kSortLds) {
+        if (p < n) heap_sort(rev + offset[p], count[p]);
+        return;
+    }
+    for (int e = threadIdx.x; e < span; e += kBlock) s_seg[e] = rev[begin + e];
+    __syncthreads();
+    if (p < n) {
+        int* a = s_seg + (offset[p] - begin);
+        const int len = count[p];
+        if (len <= 32) {
+            for (int i = 1; i < len; ++i) {
+                const int v = a[i];
+                int j = i - 1;
+                while (j >= 0 && a[j] > v) {
+                    a[j + 1] = a[j];
+                    --j;
+                }
+                a[j + 1] = v;
+            }
+        } else {
+            heap_sort(a, len);
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < span; e += kBlock) rev[begin + e] = s_seg[e];
 }
 
 // g_m_p = w_p S_p - lrd_p^2 sum_{i : p in N(i)} a_i
