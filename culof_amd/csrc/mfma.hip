@@ -309,6 +309,9 @@ struct MfmaArgs {
     int ov_cap;
     int* big_rows;       // collect mode: locals with more than kSelectMax entries
     int* big_count;
+    int* list_rows;      // collect mode: rows for the seeded list pass
+    int* list_count;
+    int list_cap;
     int* nbr_idx;
     float* nbr_d2;
     float* kdist2;
@@ -800,8 +803,11 @@ collect_select_kernel(MfmaArgs p) {
     if (__any(mine > p.cap)) {
         if (lane == 0) {
             const int at = atomicAdd(p.ov_count, 1);
+            const int at2 = at < p.ov_cap ? 0 : atomicAdd(p.list_count, 1);
             if (at < p.ov_cap) {
                 p.ov_rows[at] = qi;
+            } else if (at2 < p.list_cap) {
+                p.list_rows[at2] = qi;
             } else {
                 p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
             }
@@ -872,7 +878,16 @@ big_select_kernel(MfmaArgs p, const int* __restrict__ rows_list, const int* __re
         }
         __syncthreads();
         if (s_pre[0] < 0 || s_pre[nb] < p.k) {
-            if (threadIdx.x == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+            // A slice buffer overflowed: the seeded list sweep redoes the row
+            // when a queue is given (radix when it is full or absent).
+            if (threadIdx.x == 0) {
+                const int at = (s_pre[0] < 0 && p.ov_rows) ? atomicAdd(p.ov_count, 1) : p.ov_cap;
+                if (at < p.ov_cap) {
+                    p.ov_rows[at] = qi;
+                } else {
+                    p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+                }
+            }
             continue;
         }
         const int total = s_pre[nb];
@@ -1116,9 +1131,13 @@ int collect_cap(int) { return kSelectMax / 2; }
 // rows whose slice buffer overflows (at most kSecondRows of them) are
 // collected again with kSecondCap per slice, then go to the block select.
 // The second pass serves few rows, so it splits the candidates into many
-// short slices to keep its latency low.
+// short slices to keep its latency low. Rows that overflow it too (candidate
+// sets of many thousands, e.g. outliers in high dimension, whose distances
+// concentrate) go to the seeded list sweep over as many slices (kListRows of
+// them), which keeps no candidate buffer at all.
 constexpr int kSecondRows = 1024;
 constexpr int kSecondCap = 4096;
+constexpr int kListRows = 8192;
 int second_slices(int n_points) {
     const int tiles = (n_points + 31) / 32;
     return std::max(1, std::min(32, tiles / 8));
@@ -1136,7 +1155,9 @@ size_t mfma_scratch_bytes(int n_points, int dp, int form, int rows, int k, bool 
         const size_t second = static_cast<size_t>(std::min(rows, kSecondRows)) * second_slices(n_points);
         lists = slots * kSelectMax * sizeof(uint32_t) + slots * sizeof(int) +
                 static_cast<size_t>(rows) * 2 * sizeof(int) + 4 * sizeof(int) +
-                second * kSecondCap * sizeof(uint32_t) + second * sizeof(int) + 8 * 256;
+                second * kSecondCap * sizeof(uint32_t) + second * sizeof(int) +
+                static_cast<size_t>(std::min(rows, kListRows)) * (sizeof(int) +
+                    sizeof(uint64_t) * second_slices(n_points) * k) + 16 * 256;
     } else if (s > 1) {
         lists = sizeof(uint64_t) * s * static_cast<size_t>(rows) * k;
     }
@@ -1170,9 +1191,10 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
     MfmaArgs args{xa, centred, keyrows, ord ? ord->xs_ord : keyrows, ord ? ord->order : nullptr,
                   qlist, tseed, madj, norms, guard, n_points, k, c, row0, rows,
                   slices, nullptr, collect ? 1 : 0, nullptr, nullptr, nullptr, kSelectMax,
-                  nullptr, nullptr, 0, nullptr, nullptr, nbr_idx, nbr_d2, kdist2, fallback_rows,
-                  fallback_count};
+                  nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr, 0, nbr_idx, nbr_d2,
+                  kdist2, fallback_rows, fallback_count};
     MfmaArgs second = args;
+    MfmaArgs list = args;
     if (collect) {
         const size_t slots = static_cast<size_t>(rows) * slices;
         args.cbuf = static_cast<uint32_t*>(take(slots * kSelectMax * sizeof(uint32_t)));
@@ -1197,9 +1219,26 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
         second.ccount = static_cast<int*>(take(slots2 * sizeof(int)));
         second.big_rows = nullptr;
         second.big_count = nullptr;
-        second.ov_rows = nullptr;
-        second.ov_count = nullptr;
-        second.ov_cap = 0;
+        // Its overflow queue feeds the list pass.
+        second.ov_cap = std::min(rows, kListRows);
+        second.ov_rows = static_cast<int*>(take(static_cast<size_t>(second.ov_cap) * sizeof(int)));
+        second.ov_count = counters + 2;
+        list = args;
+        list.collect = 0;
+        list.qlist = second.ov_rows;
+        list.row_count = second.ov_count;
+        list.row0 = 0;
+        list.rows = second.ov_cap;
+        list.slices = second.slices;
+        list.partial = list.slices > 1 ? static_cast<uint64_t*>(take(
+                           sizeof(uint64_t) * list.slices * static_cast<size_t>(list.rows) * k))
+                                       : nullptr;
+        list.cbuf = nullptr;
+        list.ccount = nullptr;
+        // First-pass rows that find the second pass's queue full go there too.
+        args.list_rows = second.ov_rows;
+        args.list_count = second.ov_count;
+        args.list_cap = second.ov_cap;
     } else if (slices > 1) {
         args.partial = static_cast<uint64_t*>(take(sizeof(uint64_t) * slices * static_cast<size_t>(rows) * k));
     }
@@ -1227,7 +1266,10 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
         CULOF_CHECK(hipGetLastError());
     };
     run(args);
-    if (collect) run(second);
+    if (collect) {
+        run(second);
+        run(list);
+    }
 }
 
 }  // namespace detail
