@@ -35,6 +35,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kCap = 2048;          // band entries held in LDS between compactions
 constexpr int kRound = 4 * kBlock;  // candidates examined per round
+constexpr int kMaxTieStride = 1024; // query row staged in LDS (floats)
 
 __device__ __forceinline__ uint64_t word_of(float key, int idx) {
     const uint32_t kb = __float_as_uint(key);
@@ -143,6 +144,7 @@ tie_resolve_kernel(const float* __restrict__ xs, int stride, int n, int d, int k
                    float* __restrict__ kdist2) {
     __shared__ Band b;
     __shared__ double s_bound;
+    __shared__ __attribute__((aligned(16))) float s_q[kMaxTieStride];
     const size_t ns = static_cast<size_t>(n);
     const int queued = *tie_count;
     for (int q = blockIdx.x; q < queued; q += gridDim.x) {
@@ -166,15 +168,29 @@ tie_resolve_kernel(const float* __restrict__ xs, int stride, int n, int d, int k
         __syncthreads();
         const double bound = s_bound;
         const float* xi = xs + static_cast<size_t>(i) * stride;
+        for (int t = threadIdx.x; t < stride; t += kBlock) s_q[t] = xi[t];
+        __syncthreads();
+        const int d4 = (d + 3) / 4;  // rows are zero-padded to stride (a multiple of 4)
         for (int base = 0; base < n; base += kRound) {
 #pragma unroll 1
             for (int u = 0; u < kRound / kBlock; ++u) {
                 const int j = base + u * kBlock + static_cast<int>(threadIdx.x);
                 if (j >= n || j == i) continue;
                 const float* xj = xs + static_cast<size_t>(j) * stride;
+                // FP32 key, features in ascending order (padding adds exact zeros)
                 float e = 0.0f;
-                for (int t = 0; t < d; ++t) {
-                    const float df = xi[t] - xj[t];
+                const float4* q4 = reinterpret_cast<const float4*>(s_q);
+                const float4* c4 = reinterpret_cast<const float4*>(xj);
+                for (int v = 0; v < d4; ++v) {
+                    const float4 a = q4[v];
+                    const float4 b = c4[v];
+                    float df = a.x - b.x;
+                    e = __builtin_fmaf(df, df, e);
+                    df = a.y - b.y;
+                    e = __builtin_fmaf(df, df, e);
+                    df = a.z - b.z;
+                    e = __builtin_fmaf(df, df, e);
+                    df = a.w - b.w;
                     e = __builtin_fmaf(df, df, e);
                 }
                 if (!(static_cast<double>(e) <= bound)) continue;  // NaN stays out
@@ -228,6 +244,8 @@ void tie_break_rows(const float* keyrows, int stride, int n, int d, int k, int r
                     int* tie_rows, int* tie_count, hipStream_t s) {
     if (rows <= 0) return;
     if (k > kBlock) throw std::invalid_argument("FP64 tie-break supports k <= 256");
+    if (stride > kMaxTieStride || stride % 4 != 0)
+        throw std::invalid_argument("FP64 tie-break supports n_dims <= 1024");
     const double rho = tie_band_ratio(d);
     ProfileScope prof("tie_break", s);
     CULOF_CHECK(hipMemsetAsync(tie_count, 0, sizeof(int), s));
