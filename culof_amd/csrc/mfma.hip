@@ -737,16 +737,44 @@ __device__ void select_words(const MfmaArgs& p, const uint64_t* s_words, int tot
         const int e = lane + 64 * u;
         w[u] = e < total ? s_words[e] : kSentinel;
     }
-    // ans = the largest word with fewer than k words below it: the k-th smallest.
-    uint64_t ans = 0;
-    for (int b = 63; b >= 0; --b) {
-        const uint64_t probe = ans | (1ull << b);
+    // The k-th smallest word: the k-th key by a bitwise search on the key
+    // bits, then, only when that key is tied, the right index among the tied.
+    uint32_t kh = 0;
+    for (int b = 31; b >= 0; --b) {
+        const uint32_t probe = kh | (1u << b);
         int below = 0;
 #pragma unroll
-        for (int u = 0; u < WPL; ++u) below += __popcll(__ballot(w[u] < probe));
-        if (below < p.k) ans = probe;
+        for (int u = 0; u < WPL; ++u) below += __popcll(__ballot(static_cast<uint32_t>(w[u] >> 32) < probe));
+        if (below < p.k) kh = probe;
     }
-    if (static_cast<uint32_t>(ans >> 32) >= kInfBits) {
+    int below = 0, tied = 0;
+#pragma unroll
+    for (int u = 0; u < WPL; ++u) {
+        const uint32_t hi = static_cast<uint32_t>(w[u] >> 32);
+        below += __popcll(__ballot(hi < kh));
+        tied += __popcll(__ballot(hi == kh));
+    }
+    const int need = p.k - below;  // 1 <= need <= tied
+    uint32_t lo = 0;
+    if (need < tied) {
+        for (int b = 31; b >= 0; --b) {
+            const uint32_t probe = lo | (1u << b);
+            int cnt = 0;
+#pragma unroll
+            for (int u = 0; u < WPL; ++u)
+                cnt += __popcll(__ballot(static_cast<uint32_t>(w[u] >> 32) == kh && static_cast<uint32_t>(w[u]) < probe));
+            if (cnt < need) lo = probe;
+        }
+    } else {
+        // every tied word is taken: the k-th is the largest of them
+#pragma unroll
+        for (int u = 0; u < WPL; ++u)
+            if (static_cast<uint32_t>(w[u] >> 32) == kh) lo = max(lo, static_cast<uint32_t>(w[u]));
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1) lo = max(lo, static_cast<uint32_t>(__shfl_xor(static_cast<int>(lo), m, 64)));
+    }
+    const uint64_t ans = (static_cast<uint64_t>(kh) << 32) | lo;
+    if (kh >= kInfBits) {
         if (lane == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
         return;
     }
