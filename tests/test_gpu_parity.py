@@ -237,3 +237,23 @@ def test_order_switch_gives_same_bits(monkeypatch):
     plain = forward(X, 20)
     for key in default:
         np.testing.assert_array_equal(default[key], plain[key], err_msg=key)
+
+
+def test_every_fallback_tier():
+    """10,000 copies of one point among 150,000: their collected sets overflow
+    the first pass, the second pass and the list queue, so the rows run
+    through every tier (block select, second collect pass, seeded list pass,
+    radix). Sampled rows, duplicates included, are bit-exact with the oracle."""
+    from _gpu import knn
+
+    rng = np.random.default_rng(71)
+    X = rng.standard_normal((150_000, 4)).astype(np.float32)
+    dup = rng.choice(150_000, 10_000, replace=False)
+    X[dup] = X[dup[0]]
+    k = 12
+    gi, gd, gk = knn(X, k)
+    rows = np.r_[dup[:12], rng.choice(150_000, 12, replace=False)]
+    for r in rows:
+        oi, od, ok = oracle.knn_rows(X, k, int(r), 1)
+        np.testing.assert_array_equal(gi[:, r], oi[0], err_msg=f"row {r}")
+        _assert_same_bits(gk[r], ok[0], f"row {r} kdist2")
