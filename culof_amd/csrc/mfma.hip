@@ -802,7 +802,6 @@ template <int DP>
 __global__ void __launch_bounds__(256)
 collect_select_kernel(MfmaArgs p) {
     __shared__ __attribute__((aligned(16))) float s_q[4][DP];
-    __shared__ uint32_t s_list[4][kSelectMax];
     __shared__ uint64_t s_words[4][kSelectMax];
     __shared__ uint64_t s_sel[4][64];
     __shared__ int s_pre[4][33];
@@ -850,15 +849,14 @@ collect_select_kernel(MfmaArgs p) {
         if (lane == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
         return;
     }
+    // Exact keys straight from the slice buffers, slice by slice.
     for (int b = 0; b < nb; ++b) {
         const int b0 = s_pre[wave][b];
         const int cb = s_pre[wave][b + 1] - b0;
         const uint32_t* src = p.cbuf + (base + b) * p.cap;
-        for (int i = lane; i < cb; i += 64) s_list[wave][b0 + i] = src[i];
-    }
-    wave_lds_sync();
 #pragma unroll 1
-    for (int e = lane; e < total; e += 64) s_words[wave][e] = collected_word<DP>(p, s_q[wave], s_list[wave][e], qi);
+        for (int i = lane; i < cb; i += 64) s_words[wave][b0 + i] = collected_word<DP>(p, s_q[wave], src[i], qi);
+    }
     wave_lds_sync();
     if (total <= 128) {
         select_words<2>(p, s_words[wave], total, qi, lane, s_sel[wave]);
