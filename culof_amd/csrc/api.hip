@@ -251,8 +251,19 @@ void knn_rows(const Prepared& p, int n, int d, int k, int form, int row0, int ro
         knn_rows(p, n, d, k + 1, screen, row0, rows, method, idx1, d21, kd1, scratch, s);
         int* tie_rows = scratch.get<int>(static_cast<size_t>(rows));
         int* tie_count = scratch.get<int>(1);
+        // Near-tie rows are collected on the matrix cores like the main pass
+        // when that is the path serving it; the brute-force scan takes the rest.
+        TieBand band{};
+        const int cap = tie_band_capacity(n, d, p.stride, screen, rows);
+        if (method == 0 && cap > 0 && choose_path(d, k + 1, screen, p.stride) == Path::kMfma) {
+            band = TieBand{p.xc, p.norms, p.guard, screen, cap,
+                           scratch.get<int>(static_cast<size_t>(cap)), scratch.get<int>(1),
+                           scratch.get<double>(static_cast<size_t>(cap)),
+                           scratch.get<float>(static_cast<size_t>(n)),
+                           scratch.get<unsigned char>(tie_band_scratch_bytes(n, d, screen, cap))};
+        }
         tie_break_rows(p.xs, p.stride, n, d, k, row0, rows, idx1, d21, nbr_idx, nbr_d2, kdist2,
-                       tie_rows, tie_count, s);
+                       tie_rows, tie_count, band.rows ? &band : nullptr, s);
         return;
     }
     const int tile = tile_rows_for(rows);

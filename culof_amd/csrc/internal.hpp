@@ -132,11 +132,37 @@ void seed_thresholds(const CandidateOrder& o, int n, int k, float* tseed, hipStr
 // ---- ties.hip: FP64 tie-break over a (k+1)-neighbour result ----
 // Relative band (1 + g') / (1 - g') of FP32 keys that FP64 may reorder.
 double tie_band_ratio(int d);
+// Near-tie rows settled through the matrix-core collect pass (mfma.hip): a
+// queue of rows with their band bounds and the buffers of that pass.
+struct TieBand {
+    const float* centred;  // centred rows (stride = the key rows' stride)
+    const float* norms;
+    const int* guard;
+    int form;              // screen form: kFormPlain or kFormBF16
+    int cap;               // rows the queue holds (tie_band_capacity)
+    int* rows;             // queued original row indices, cap ints
+    int* count;            // queue length; rows past cap went to the brute-force queue
+    double* bound;         // per queue slot: the band bound K rho
+    float* tband;          // per original row (n floats): smallest float >= its bound
+    void* scratch;         // tie_band_scratch_bytes
+};
+// Rows the band queue holds for a call over `rows` rows; 0 when the collect
+// pass does not serve (n_dims, stride, form).
+int tie_band_capacity(int n_points, int n_dims, int stride, int form, int rows);
+size_t tie_band_scratch_bytes(int n_points, int n_dims, int form, int cap);
+// Collect pass over the queued rows with their bounds as thresholds, then the
+// FP64 band select; rows it cannot hold are appended to brute_rows.
+void tie_band_select(const float* keyrows, int stride, int n_points, int k, const TieBand& band,
+                     int* nbr_idx, float* nbr_d2, float* kdist2, int* brute_rows, int* brute_count,
+                     hipStream_t stream);
+
 // idx1 / d21: the (k+1)-lists of rows [row0, row0 + rows) (stride n). Writes
 // the k-lists of those rows; tie_rows needs `rows` ints, tie_count one int.
+// With `band`, near-tie rows go through tie_band_select first and only the
+// rows it cannot hold are scanned by brute force.
 void tie_break_rows(const float* keyrows, int stride, int n, int d, int k, int row0, int rows,
                     const int* idx1, const float* d21, int* nbr_idx, float* nbr_d2, float* kdist2,
-                    int* tie_rows, int* tie_count, hipStream_t s);
+                    int* tie_rows, int* tie_count, const TieBand* band, hipStream_t s);
 
 // ---- backward.hip: d(sum w LOF)/dX with kNN sets and max() branches fixed ----
 size_t backward_scratch_bytes(int n, int d, int k);

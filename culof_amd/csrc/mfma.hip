@@ -300,9 +300,9 @@ struct MfmaArgs {
     int slices;
     uint64_t* partial;   // [slices][rows][k] when slices > 1
     int collect;         // 1: collect kernels (needs tseed), 0: list kernels
-    const int* row_count;  // optional device count bounding rows (list kernels)
-    uint32_t* cbuf;      // collect mode: [rows][slices][2][cap] candidate positions
-    int* ccount;         // collect mode: [rows][slices][2] entries (> cap: overflowed)
+    const int* row_count;  // optional device count bounding rows
+    uint32_t* cbuf;      // collect mode: [rows][slices][cap] candidate positions
+    int* ccount;         // collect mode: [rows][slices] entries (> cap: overflowed)
     int cap;
     int* ov_rows;        // collect mode: rows whose slice buffer overflowed (second pass)
     int* ov_count;
@@ -960,6 +960,121 @@ big_select_kernel(MfmaArgs p, const int* __restrict__ rows_list, const int* __re
     }
 }
 
+// ---- FP64 band select over a collect pass (near-tie rows) ----
+//
+// A near-tie row (its k-th and (k+1)-th FP32 keys closer than FP32 can tell
+// apart) is settled on FP64 keys over its band {j : e32(i, j) <= bound}; the
+// collect pass ran with a threshold of at least `bound`, so the band is among
+// the collected entries. The block keys them in FP32, keeps the band, re-keys
+// it in FP64 (separate multiply and add, ascending feature order) and emits
+// the k smallest (e64, index) pairs in the reference's slot order: below the
+// k-th FP64 key in ascending index, then the tied ones in ascending index.
+// nbr_d2 / kdist2 keep the FP32 keys of the chosen pairs. Rows whose slice
+// buffer overflowed or whose collected set exceeds kBandMax go to the
+// brute-force queue (the whole candidate range, same rule).
+constexpr int kBandMax = 2048;  // collected entries one block holds
+
+template <int DP>
+__global__ void __launch_bounds__(256)
+band_select_kernel(MfmaArgs p, const double* __restrict__ bound_of, int* __restrict__ brute_rows,
+                   int* __restrict__ brute_count) {
+    __shared__ __attribute__((aligned(16))) float s_q[DP];
+    __shared__ uint64_t s_e64[kBandMax];  // non-negative doubles order as integers
+    __shared__ int s_idx[kBandMax];
+    __shared__ float s_e32[kBandMax];
+    __shared__ int s_rank[kBandMax];
+    __shared__ int s_pre[33];
+    __shared__ int s_m;
+    __shared__ uint64_t s_kth;
+    __shared__ float s_kd;
+    if (*p.guard) return;  // the collect pass queued every row as brute force
+    const int count = min(*p.row_count, p.rows);
+    const int nb = p.slices;
+    const size_t ns = static_cast<size_t>(p.n);
+    for (int r = blockIdx.x; r < count; r += gridDim.x) {
+        const int qi = p.qlist[r];
+        const double bound = bound_of[r];
+        const size_t base = static_cast<size_t>(r) * nb;
+        __syncthreads();
+        if (threadIdx.x < nb) s_pre[threadIdx.x + 1] = p.ccount[base + threadIdx.x];
+        for (int t = threadIdx.x; t < DP; t += 256) s_q[t] = p.xs[static_cast<size_t>(qi) * DP + t];
+        if (threadIdx.x == 0) s_m = 0;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int run = 0;
+            bool over = false;
+            s_pre[0] = 0;
+            for (int b = 0; b < nb; ++b) {
+                over |= s_pre[b + 1] > p.cap;
+                run += s_pre[b + 1];
+                s_pre[b + 1] = run;
+            }
+            if (over || run > kBandMax) s_pre[0] = -1;
+        }
+        __syncthreads();
+        if (s_pre[0] < 0) {
+            if (threadIdx.x == 0) brute_rows[atomicAdd(brute_count, 1)] = qi;
+            continue;
+        }
+        const int total = s_pre[nb];
+        for (int e = threadIdx.x; e < total; e += 256) {
+            int b = 0;
+            while (e >= s_pre[b + 1]) ++b;
+            const uint32_t pos = p.cbuf[(base + b) * p.cap + (e - s_pre[b])];
+            const int orig = p.cand ? p.cand[pos] : static_cast<int>(pos);
+            if (orig == qi) continue;
+            const float* row = p.xs_cand + static_cast<size_t>(pos) * DP;
+            const float e32 = exact_key_q4<DP>(s_q, row);
+            if (!(static_cast<double>(e32) <= bound)) continue;  // NaN stays out
+            double e64 = 0.0;
+            for (int t = 0; t < DP; ++t) {  // zero padding adds exact zeros
+                const double df = __dsub_rn(static_cast<double>(s_q[t]), static_cast<double>(row[t]));
+                e64 = __dadd_rn(e64, __dmul_rn(df, df));
+            }
+            const int at = atomicAdd(&s_m, 1);
+            s_e64[at] = static_cast<uint64_t>(__double_as_longlong(e64));
+            s_idx[at] = orig;
+            s_e32[at] = e32;
+        }
+        __syncthreads();
+        const int m = s_m;
+        if (m < p.k) {  // cannot happen (the FP32 k-list lies in the band); stay exact regardless
+            if (threadIdx.x == 0) brute_rows[atomicAdd(brute_count, 1)] = qi;
+            continue;
+        }
+        // Rank of each entry in (e64, index) order; the k-th is the one of rank k - 1.
+        for (int t = threadIdx.x; t < m; t += 256) {
+            const uint64_t et = s_e64[t];
+            const int it = s_idx[t];
+            int rank = 0;
+            for (int u = 0; u < m; ++u) {
+                const uint64_t eu = s_e64[u];
+                rank += eu < et || (eu == et && s_idx[u] < it);
+            }
+            s_rank[t] = rank;
+            if (rank == p.k - 1) {
+                s_kth = et;
+                s_kd = s_e32[t];
+            }
+        }
+        __syncthreads();
+        const uint64_t kth = s_kth;
+        for (int t = threadIdx.x; t < m; t += 256) {
+            const int rank = s_rank[t];
+            if (rank >= p.k) continue;
+            int slot = rank;  // ties of the k-th sit at ranks [below, k) in index order
+            if (s_e64[t] < kth) {
+                slot = 0;
+                const int it = s_idx[t];
+                for (int u = 0; u < m; ++u) slot += s_e64[u] < kth && s_idx[u] < it;
+            }
+            p.nbr_idx[static_cast<size_t>(slot) * ns + qi] = s_idx[t];
+            p.nbr_d2[static_cast<size_t>(slot) * ns + qi] = s_e32[t];
+        }
+        if (threadIdx.x == 0) p.kdist2[qi] = s_kd;
+    }
+}
+
 // One thread per query: fold the slices' partial lists, then emit.
 template <int KMAX>
 __global__ void __launch_bounds__(64)
@@ -1102,27 +1217,43 @@ bool dispatch_k(const MfmaArgs* a, int k, hipStream_t stream) {
     return false;
 }
 
-template <class Op>
-bool dispatch(int dp, int k, const MfmaArgs* a, hipStream_t stream) {
+// Calls f(std::integral_constant<int, DP>) for the padded widths Op is built for.
+template <class Op, class F>
+bool for_dp(int dp, F&& f) {
+    using std::integral_constant;
     if constexpr (std::is_same_v<Op, ScreenF32>) {
         switch (dp) {
-            case 4: return dispatch_k<Op, 4>(a, k, stream);
-            case 8: return dispatch_k<Op, 8>(a, k, stream);
-            case 16: return dispatch_k<Op, 16>(a, k, stream);
-            case 32: return dispatch_k<Op, 32>(a, k, stream);
-            case 64: return dispatch_k<Op, 64>(a, k, stream);
-            case 128: return dispatch_k<Op, 128>(a, k, stream);
-            default: return false;
-        }
-    } else {
-        switch (dp) {
-            case 16: return dispatch_k<Op, 16>(a, k, stream);
-            case 32: return dispatch_k<Op, 32>(a, k, stream);
-            case 64: return dispatch_k<Op, 64>(a, k, stream);
-            case 128: return dispatch_k<Op, 128>(a, k, stream);
-            default: return false;
+            case 4: return f(integral_constant<int, 4>{});
+            case 8: return f(integral_constant<int, 8>{});
+            default: break;
         }
     }
+    switch (dp) {
+        case 16: return f(integral_constant<int, 16>{});
+        case 32: return f(integral_constant<int, 32>{});
+        case 64: return f(integral_constant<int, 64>{});
+        case 128: return f(integral_constant<int, 128>{});
+        default: return false;
+    }
+}
+
+template <class Op>
+bool dispatch(int dp, int k, const MfmaArgs* a, hipStream_t stream) {
+    return for_dp<Op>(dp, [&](auto D) { return dispatch_k<Op, decltype(D)::value>(a, k, stream); });
+}
+
+// The band pass: collect over the queued near-tie rows, then the FP64 select.
+template <class Op, int DP>
+void launch_band(const MfmaArgs& a, const double* bound, int* brute_rows, int* brute_count,
+                 hipStream_t stream) {
+    const int qtiles = (a.rows + 31) / 32;
+    const dim3 grid((qtiles + kTilesPerBlock - 1) / kTilesPerBlock, a.slices);
+    {
+        ProfileScope prof("tie_collect", stream);
+        mfma_collect_kernel<Op, DP><<<grid, 64 * kTilesPerBlock, 0, stream>>>(a);
+    }
+    ProfileScope prof("tie_band", stream);
+    band_select_kernel<DP><<<std::min(a.rows, 1024), 256, 0, stream>>>(a, bound, brute_rows, brute_count);
 }
 
 }  // namespace
@@ -1161,6 +1292,27 @@ int collect_cap(int) { return kSelectMax / 2; }
 // sets of many thousands, e.g. outliers in high dimension, whose distances
 // concentrate) go to the seeded list sweep over as many slices (kListRows of
 // them), which keeps no candidate buffer at all.
+// Screen operand rows (form layout) and adjusted norms, in candidate order
+// (cand: position -> original index; null = natural order).
+void prepare_screen(const float* centred, const float* norms, const int* cand, int n_points, int dp,
+                    bool bf16, float c, void* xa, float* madj, hipStream_t stream) {
+    ProfileScope prof("screen_prep", stream);
+    const int padded_rows = (n_points + 31) / 32 * 32;
+    const size_t padded_cells = static_cast<size_t>(padded_rows) * dp;
+    const unsigned grid = static_cast<unsigned>((padded_cells + 255) / 256);
+    if (bf16) {
+        screen_rows_bf16_kernel<<<grid, 256, 0, stream>>>(centred, cand, n_points, dp, padded_cells,
+                                                            static_cast<__bf16*>(xa));
+    } else {
+        screen_rows_f32_kernel<<<grid, 256, 0, stream>>>(centred, cand, n_points, padded_rows, dp,
+                                                           static_cast<float*>(xa));
+    }
+    CULOF_CHECK(hipGetLastError());
+    adjusted_norms_kernel<<<(padded_rows + 255) / 256, 256, 0, stream>>>(norms, cand, n_points,
+                                                                          padded_rows, c, madj);
+    CULOF_CHECK(hipGetLastError());
+}
+
 constexpr int kSecondRows = 1024;
 constexpr int kSecondCap = 4096;
 constexpr int kListRows = 8192;
@@ -1269,22 +1421,7 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
         args.partial = static_cast<uint64_t*>(take(sizeof(uint64_t) * slices * static_cast<size_t>(rows) * k));
     }
 
-    {
-        ProfileScope prof("screen_prep", stream);
-        const unsigned grid = static_cast<unsigned>((padded_cells + 255) / 256);
-        const int* cand = ord ? ord->order : nullptr;
-        if (bf16) {
-            screen_rows_bf16_kernel<<<grid, 256, 0, stream>>>(centred, cand, n_points, dp,
-                                                                padded_cells, static_cast<__bf16*>(xa));
-        } else {
-            screen_rows_f32_kernel<<<grid, 256, 0, stream>>>(centred, cand, n_points, padded_rows, dp,
-                                                               static_cast<float*>(xa));
-        }
-        CULOF_CHECK(hipGetLastError());
-        adjusted_norms_kernel<<<(padded_rows + 255) / 256, 256, 0, stream>>>(norms, cand, n_points,
-                                                                              padded_rows, c, madj);
-        CULOF_CHECK(hipGetLastError());
-    }
+    prepare_screen(centred, norms, ord ? ord->order : nullptr, n_points, dp, bf16, c, xa, madj, stream);
     auto run = [&](const MfmaArgs& a) {
         const bool ok = bf16 ? dispatch<ScreenBF16>(dp, k, &a, stream)
                              : dispatch<ScreenF32>(dp, k, &a, stream);
@@ -1296,6 +1433,65 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
         run(second);
         run(list);
     }
+}
+
+// Band pass for near-tie rows: up to kBandRows queued rows, collected in
+// natural candidate order over second_slices(n) slices of kBandCap entries.
+// A band is a few entries beyond k, so the buffers are small; rows that
+// overflow them take the brute-force scan.
+constexpr int kBandRows = 32768;
+constexpr int kBandCap = 128;
+
+int tie_band_capacity(int n_points, int n_dims, int stride, int form, int rows) {
+    const int dp = mfma_padded_dims(n_dims, form);
+    const bool ok = form == kFormBF16 ? for_dp<ScreenBF16>(dp, [](auto) { return true; })
+                                      : for_dp<ScreenF32>(dp, [](auto) { return true; });
+    return ok && dp == stride && n_points > 0 ? std::min(rows, kBandRows) : 0;
+}
+
+size_t tie_band_scratch_bytes(int n_points, int n_dims, int form, int cap) {
+    const int dp = mfma_padded_dims(n_dims, form);
+    const size_t padded = static_cast<size_t>((n_points + 31) / 32) * 32;
+    const size_t slots = static_cast<size_t>(cap) * second_slices(n_points);
+    return padded * dp * 4 + 256 + padded * sizeof(float) + 256 + slots * kBandCap * sizeof(uint32_t) +
+           256 + slots * sizeof(int) + 256;
+}
+
+void tie_band_select(const float* keyrows, int stride, int n_points, int k, const TieBand& band,
+                     int* nbr_idx, float* nbr_d2, float* kdist2, int* brute_rows, int* brute_count,
+                     hipStream_t stream) {
+    const int dp = stride;
+    const bool bf16 = band.form == kFormBF16;
+    const float c = bf16 ? ScreenBF16::slack(dp) : ScreenF32::slack(dp);
+    const int padded_rows = (n_points + 31) / 32 * 32;
+    unsigned char* base = static_cast<unsigned char*>(band.scratch);
+    auto take = [&](size_t bytes) {
+        void* r = base;
+        base += (bytes + 255) / 256 * 256;
+        return r;
+    };
+    void* xa = take(static_cast<size_t>(padded_rows) * dp * 4);
+    float* madj = static_cast<float*>(take(sizeof(float) * padded_rows));
+    const int slices = second_slices(n_points);
+    const size_t slots = static_cast<size_t>(band.cap) * slices;
+    uint32_t* cbuf = static_cast<uint32_t*>(take(slots * kBandCap * sizeof(uint32_t)));
+    int* ccount = static_cast<int*>(take(slots * sizeof(int)));
+    prepare_screen(band.centred, band.norms, nullptr, n_points, dp, bf16, c, xa, madj, stream);
+    MfmaArgs a{xa, band.centred, keyrows, keyrows, nullptr, band.rows, band.tband, madj, band.norms,
+               band.guard, n_points, k, c, 0, band.cap, slices, nullptr, 1, band.count, cbuf, ccount,
+               kBandCap, nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr, 0, nbr_idx, nbr_d2,
+               kdist2, brute_rows, brute_count};
+    auto go = [&](auto D) {
+        if (bf16) {
+            launch_band<ScreenBF16, decltype(D)::value>(a, band.bound, brute_rows, brute_count, stream);
+        } else {
+            launch_band<ScreenF32, decltype(D)::value>(a, band.bound, brute_rows, brute_count, stream);
+        }
+        return true;
+    };
+    const bool ok = bf16 ? for_dp<ScreenBF16>(dp, go) : for_dp<ScreenF32>(dp, go);
+    if (!ok) throw std::invalid_argument("tie band: unsupported (n_dims, form)");
+    CULOF_CHECK(hipGetLastError());
 }
 
 }  // namespace detail

@@ -42,12 +42,19 @@ __device__ __forceinline__ uint64_t word_of(float key, int idx) {
     return (static_cast<uint64_t>(kb > kInfBits ? kInfBits : kb) << 32) | static_cast<uint32_t>(idx);
 }
 
-// Split the (k+1)-lists: emit rows that are not near-ties, queue the rest.
+// Smallest float >= b (b >= 0; +inf past FLT_MAX).
+__device__ __forceinline__ float float_at_least(double b) {
+    const float f = static_cast<float>(b);
+    return static_cast<double>(f) < b ? __uint_as_float(__float_as_uint(f) + 1u) : f;
+}
+
+// Split the (k+1)-lists: emit rows that are not near-ties, queue the rest
+// (to the band queue while it has room, else to the brute-force queue).
 __global__ void __launch_bounds__(kBlock)
 tie_split_kernel(const int* __restrict__ idx1, const float* __restrict__ d21, int n, int row0,
                  int rows, int k, double rho, int* __restrict__ nbr_idx,
                  float* __restrict__ nbr_d2, float* __restrict__ kdist2, int* __restrict__ tie_rows,
-                 int* __restrict__ tie_count) {
+                 int* __restrict__ tie_count, TieBand band) {
     const int r = blockIdx.x * kBlock + threadIdx.x;
     if (r >= rows) return;
     const int i = row0 + r;
@@ -69,8 +76,16 @@ tie_split_kernel(const int* __restrict__ idx1, const float* __restrict__ d21, in
     }
     const float K = __uint_as_float(static_cast<uint32_t>(wk >> 32));
     const float K1 = __uint_as_float(static_cast<uint32_t>(wmax >> 32));
-    if (isfinite(K) && static_cast<double>(K1) <= static_cast<double>(K) * rho) {
-        tie_rows[atomicAdd(tie_count, 1)] = i;
+    const double bound = static_cast<double>(K) * rho;
+    if (isfinite(K) && static_cast<double>(K1) <= bound) {
+        const int at = band.rows ? atomicAdd(band.count, 1) : band.cap;
+        if (at < band.cap) {
+            band.rows[at] = i;
+            band.bound[at] = bound;
+            band.tband[i] = float_at_least(bound);
+        } else {
+            tie_rows[atomicAdd(tie_count, 1)] = i;
+        }
         return;
     }
     // Keys below K keep their ascending-index order; keys equal to K follow.
@@ -241,7 +256,7 @@ double tie_band_ratio(int d) {
 
 void tie_break_rows(const float* keyrows, int stride, int n, int d, int k, int row0, int rows,
                     const int* idx1, const float* d21, int* nbr_idx, float* nbr_d2, float* kdist2,
-                    int* tie_rows, int* tie_count, hipStream_t s) {
+                    int* tie_rows, int* tie_count, const TieBand* band, hipStream_t s) {
     if (rows <= 0) return;
     if (k > kBlock) throw std::invalid_argument("FP64 tie-break supports k <= 256");
     if (stride > kMaxTieStride || stride % 4 != 0)
@@ -249,8 +264,17 @@ void tie_break_rows(const float* keyrows, int stride, int n, int d, int k, int r
     const double rho = tie_band_ratio(d);
     ProfileScope prof("tie_break", s);
     CULOF_CHECK(hipMemsetAsync(tie_count, 0, sizeof(int), s));
+    TieBand queue{};
+    if (band && band->cap > 0) {
+        queue = *band;
+        CULOF_CHECK(hipMemsetAsync(queue.count, 0, sizeof(int), s));
+    }
     tie_split_kernel<<<(rows + kBlock - 1) / kBlock, kBlock, 0, s>>>(
-        idx1, d21, n, row0, rows, k, rho, nbr_idx, nbr_d2, kdist2, tie_rows, tie_count);
+        idx1, d21, n, row0, rows, k, rho, nbr_idx, nbr_d2, kdist2, tie_rows, tie_count, queue);
+    CULOF_CHECK(hipGetLastError());
+    if (queue.rows) {
+        tie_band_select(keyrows, stride, n, k, queue, nbr_idx, nbr_d2, kdist2, tie_rows, tie_count, s);
+    }
     const int grid = rows < 1024 ? rows : 1024;
     tie_resolve_kernel<<<grid, kBlock, 0, s>>>(keyrows, stride, n, d, k, rho, idx1, d21, tie_rows,
                                                tie_count, nbr_idx, nbr_d2, kdist2);

@@ -123,3 +123,40 @@ def test_many_exact_duplicates():
     np.testing.assert_array_equal(gi.T[:50], oi)
     np.testing.assert_array_equal(_bits(gk[:50]), _bits(ok))
     assert tie.all()
+
+
+@pytest.mark.parametrize("form", [0, 1], ids=["fp32", "bf16"])
+def test_band_pass_matches_brute_force_scan(form):
+    """Near-tie rows settled through the collect-based band pass (the matrix-core
+    path) against the brute-force scan (the radix path), every row bit for bit.
+    Integer-grid data makes almost every row a near-tie: more rows than the band
+    queue holds (the excess is scanned), bands of a few hundred entries (600
+    scattered copies of one point) and bands larger than one block holds (3,000
+    scattered copies of another)."""
+    from _gpu import knn
+    from culof_amd import _native
+
+    rng = np.random.default_rng(21)
+    n, d, k = 40_000, 4, 16
+    X = np.round(rng.standard_normal((n, d)) * 4).astype(np.float32)
+    perm = rng.permutation(n)
+    X[perm[:600]] = X[perm[0]]
+    X[perm[600:3600]] = X[perm[600]]
+    _native.profile_reset()
+    _native.profile_enable(True)
+    try:
+        bi, bd, bk = knn(X, k, form=form | TIES)
+    finally:
+        _native.profile_enable(False)
+    assert _native.profile_read("tie_band")[1] > 0
+    ri, rd, rk = knn(X, k, method=1, form=TIES)
+    np.testing.assert_array_equal(bi, ri)
+    np.testing.assert_array_equal(_bits(bd), _bits(rd))
+    np.testing.assert_array_equal(_bits(bk), _bits(rk))
+    ties = 0
+    for r in np.r_[perm[:3], perm[600:603], rng.choice(n, 6, replace=False)]:
+        oi, od, ok, tie = oracle.knn_ties(X, k, int(r), 1)
+        np.testing.assert_array_equal(bi[:, r], oi[0], err_msg=f"row {r}")
+        np.testing.assert_array_equal(_bits(bk[r]), _bits(ok[0]), err_msg=f"row {r}")
+        ties += int(tie[0])
+    assert ties >= 6
