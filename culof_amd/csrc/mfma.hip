@@ -40,6 +40,7 @@
 #include "internal.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <stdexcept>
 #include <type_traits>
 
@@ -72,6 +73,12 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // Sorted register list, descending, threshold at a[0] (see sweep.hip).
+// Lanes below this one whose bit is set in a wave-wide mask.
+__device__ __forceinline__ int lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
 template <int KMAX>
 __device__ __forceinline__ void list_init(uint64_t (&a)[KMAX], int k) {
 #pragma unroll
@@ -210,6 +217,9 @@ struct ChunkShape {
     static constexpr int kTiles = kFit >= 8 ? 8 : kFit >= 4 ? 4 : kFit >= 2 ? 2 : 1;  // a power of two
     // A operand plus the tile norms (4 bytes per row)
     static constexpr size_t kBytes = static_cast<size_t>(kTiles) * (kTileBytes + 32 * 4);
+    // the symmetric pass also stages the rows' column thresholds (4 bytes per
+    // row) and their per-(tile, half) maxima (16 bytes per tile, 8 used)
+    static constexpr size_t kSymBytes = static_cast<size_t>(kTiles) * (kTileBytes + 32 * 4 * 2 + 16);
 };
 
 // Exact key of a query row against key row xs_row: the same FMA chain as the
@@ -317,19 +327,26 @@ struct MfmaArgs {
     float* kdist2;
     int* fb_rows;
     int* fb_count;
+    const int* heavy;    // symmetric pass: 1 for rows (by position) queued for the list pass up front
 };
 
-// Streams this block's candidate slice (blockIdx.y of p.slices) in chunks of
-// CT tiles through a double-buffered LDS stage shared by the block's 4 waves:
-// while the waves screen chunk c out of one buffer, every thread holds its
-// share of chunk c + 1 in registers and writes it to the other buffer
-// afterwards; one barrier per chunk. For each tile, visit(acc, tile) gets the
-// accumulator C = m + A B of this lane (16 candidate rows, one query). The
-// operand arrays are padded to whole tiles (zero rows, NaN norms); tiles past
-// the slice end are never visited. `stage` holds 2 * ChunkShape::kBytes.
-template <class Op, int DP, class Visit>
-__device__ __forceinline__ void stream_tiles(const MfmaArgs& p, const typename Op::template Frag<DP>& bq,
-                                             int j, int h, unsigned char* stage, Visit&& visit) {
+// Streams candidate tiles [t_begin, t_end) in chunks of CT tiles through a
+// double-buffered LDS stage shared by the block's 4 waves: while the waves
+// screen chunk c out of one buffer, every thread holds its share of chunk
+// c + 1 in registers and writes it to the other buffer afterwards; one
+// barrier per chunk. Tile v is read from v, or v - wrap when v >= wrap
+// (t_end <= 2 wrap). For each tile, visit(acc, v) gets the accumulator
+// C = m + A B of this lane (16 candidate rows, one query); with kSym it is
+// visit(acc, v, wmax, wrow): also this half's maximum column threshold over
+// the tile (wmax[tile].x for half 0, .y for half 1) and the tile's staged
+// column thresholds (8 float4s: rows 4i..4i+3). The operand arrays are padded
+// to whole tiles (zero rows, NaN norms and thresholds); tiles past t_end are
+// never visited. `stage` holds 2 * ChunkShape::kBytes (kSymBytes with kSym).
+template <class Op, int DP, bool kSym, class Visit>
+__device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename Op::template Frag<DP>& bq,
+                                             int j, int h, unsigned char* stage, int t_begin, int t_end,
+                                             int wrap, const float* __restrict__ wcol,
+                                             const float2* __restrict__ wmax, Visit&& visit) {
     using Frag = typename Op::template Frag<DP>;
     using Elem = typename Op::Elem;
     using Shape = ChunkShape<Op, DP>;
@@ -344,39 +361,62 @@ __device__ __forceinline__ void stream_tiles(const MfmaArgs& p, const typename O
     constexpr int kLoadsA = (kChunkF4 + 255) / 256;  // per thread
     constexpr int kChunkM4 = kRowsPerChunk / 4;       // float4s of norms per chunk
     static_assert(DP * Op::kRowMul * sizeof(Elem) % 16 == 0, "screen rows must be whole float4s");
+    static_assert(2 * kChunkM4 + CT <= 256, "one staging thread per float4");
     float4* s_a = reinterpret_cast<float4*>(stage);
     float4* s_m = s_a + 2 * kLdsChunkF4;
+    float4* s_w = s_m + 2 * kChunkM4;              // kSym: column thresholds
+    float2* s_x = reinterpret_cast<float2*>(s_w + 2 * kChunkM4);  // kSym: per-tile maxima
 
-    const int n_tiles = (p.n + 31) / 32;
-    const int t_begin = static_cast<int>((static_cast<long long>(n_tiles) * blockIdx.y) / p.slices);
-    const int t_end = static_cast<int>((static_cast<long long>(n_tiles) * (blockIdx.y + 1)) / p.slices);
     const int n_chunks = (t_end - t_begin + CT - 1) / CT;
     const float4* ga = static_cast<const float4*>(p.xa);
     const float4* gm = reinterpret_cast<const float4*>(p.madj);
+    const float4* gw = reinterpret_cast<const float4*>(wcol);
     const float nan = __uint_as_float(0x7FC00000u);
+    auto phys = [&](int v) { return v >= wrap ? v - wrap : v; };
 
     float4 ra[kLoadsA];
     float4 rm = make_float4(nan, nan, nan, nan);
+    float2 rx = make_float2(nan, nan);
+    // threads [0, kChunkM4) stage norms, [kChunkM4, 2 kChunkM4) column
+    // thresholds, [2 kChunkM4, 2 kChunkM4 + CT) the per-tile maxima
+    const int tid = threadIdx.x;
     auto fetch = [&](int t0) {
 #pragma unroll
         for (int u = 0; u < kLoadsA; ++u) {
-            const int idx = threadIdx.x + 256 * u;
-            const bool live = idx < kChunkF4 && t0 + idx / kTileF4 < t_end;
-            ra[u] = live ? ga[static_cast<size_t>(t0) * kTileF4 + idx] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const int idx = tid + 256 * u;
+            const int vt = t0 + idx / kTileF4;
+            const bool live = idx < kChunkF4 && vt < t_end;
+            ra[u] = live ? ga[static_cast<size_t>(phys(vt)) * kTileF4 + idx % kTileF4]
+                         : make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        if (threadIdx.x < kChunkM4) {
-            const bool live = t0 + static_cast<int>(threadIdx.x) / 8 < t_end;
-            rm = live ? gm[static_cast<size_t>(t0) * 8 + threadIdx.x] : make_float4(nan, nan, nan, nan);
+        if (tid < kChunkM4) {
+            const int vt = t0 + tid / 8;
+            rm = vt < t_end ? gm[static_cast<size_t>(phys(vt)) * 8 + tid % 8] : make_float4(nan, nan, nan, nan);
+        }
+        if constexpr (kSym) {
+            if (tid >= kChunkM4 && tid < 2 * kChunkM4) {
+                const int e = tid - kChunkM4;
+                const int vt = t0 + e / 8;
+                rm = vt < t_end ? gw[static_cast<size_t>(phys(vt)) * 8 + e % 8] : make_float4(nan, nan, nan, nan);
+            }
+            if (tid >= 2 * kChunkM4 && tid < 2 * kChunkM4 + CT) {
+                const int vt = t0 + tid - 2 * kChunkM4;
+                rx = vt < t_end ? wmax[phys(vt)] : make_float2(nan, nan);
+            }
         }
     };
     auto stage_chunk = [&](int buf) {
 #pragma unroll
         for (int u = 0; u < kLoadsA; ++u) {
-            const int idx = threadIdx.x + 256 * u;
+            const int idx = tid + 256 * u;
             const int row = idx / kRowF4;
             if (idx < kChunkF4) s_a[buf * kLdsChunkF4 + row * kPitchF4 + (idx - row * kRowF4)] = ra[u];
         }
-        if (threadIdx.x < kChunkM4) s_m[buf * kChunkM4 + threadIdx.x] = rm;
+        if (tid < kChunkM4) s_m[buf * kChunkM4 + tid] = rm;
+        if constexpr (kSym) {
+            if (tid >= kChunkM4 && tid < 2 * kChunkM4) s_w[buf * kChunkM4 + tid - kChunkM4] = rm;
+            if (tid >= 2 * kChunkM4 && tid < 2 * kChunkM4 + CT) s_x[buf * CT + tid - 2 * kChunkM4] = rx;
+        }
     };
     // Accumulator initialised with the 16 adjusted norms of this lane's rows.
     auto init_acc = [&](const float4* sm, int tt) {
@@ -391,6 +431,14 @@ __device__ __forceinline__ void stream_tiles(const MfmaArgs& p, const typename O
         }
         return acc;
     };
+    float wm[kSym ? CT : 1];  // kSym: this half's column maxima of the chunk's tiles
+    auto call = [&](const floatx16& acc, int buf, int tt, int v) {
+        if constexpr (kSym) {
+            visit(acc, v, wm[tt], s_w + buf * kChunkM4 + tt * 8);
+        } else {
+            visit(acc, v);
+        }
+    };
 
     if (n_chunks > 0) {
         fetch(t_begin);
@@ -403,6 +451,13 @@ __device__ __forceinline__ void stream_tiles(const MfmaArgs& p, const typename O
         if (ch + 1 < n_chunks) fetch(t0 + CT);  // next chunk in flight
         const float4* sa = s_a + buf * kLdsChunkF4;
         const float4* sm = s_m + buf * kChunkM4;
+        if constexpr (kSym) {
+#pragma unroll
+            for (int tt = 0; tt < CT; ++tt) {
+                const float2 x = s_x[buf * CT + tt];
+                wm[tt] = h ? x.y : x.x;
+            }
+        }
         // Two tiles per step: two independent MFMA chains hide each other's
         // result latency.
 #pragma unroll
@@ -420,12 +475,22 @@ __device__ __forceinline__ void stream_tiles(const MfmaArgs& p, const typename O
             }
             acc0 = Op::template run<DP>(f0, bq, acc0);
             if (two) acc1 = Op::template run<DP>(f1, bq, acc1);
-            if (t0 + tt < t_end) visit(acc0, t0 + tt);
-            if (two && t0 + tt + 1 < t_end) visit(acc1, t0 + tt + 1);
+            if (t0 + tt < t_end) call(acc0, buf, tt, t0 + tt);
+            if (two && t0 + tt + 1 < t_end) call(acc1, buf, tt + 1, t0 + tt + 1);
         }
         if (ch + 1 < n_chunks) stage_chunk(buf ^ 1);
         __syncthreads();
     }
+}
+
+// The block's candidate slice (blockIdx.y of p.slices) of all candidate tiles.
+template <class Op, int DP, class Visit>
+__device__ __forceinline__ void stream_tiles(const MfmaArgs& p, const typename Op::template Frag<DP>& bq,
+                                             int j, int h, unsigned char* stage, Visit&& visit) {
+    const int n_tiles = (p.n + 31) / 32;
+    const int t_begin = static_cast<int>((static_cast<long long>(n_tiles) * blockIdx.y) / p.slices);
+    const int t_end = static_cast<int>((static_cast<long long>(n_tiles) * (blockIdx.y + 1)) / p.slices);
+    stream_range<Op, DP, false>(p, bq, j, h, stage, t_begin, t_end, n_tiles, nullptr, nullptr, visit);
 }
 
 // Wide lists need the whole register file: let those instantiations run at
@@ -681,15 +746,300 @@ mfma_collect_kernel(MfmaArgs p) {
     if (valid && h == 0) p.ccount[slot] = cnt;
 }
 
+// ---- symmetric collect: a full run, queries and candidates in one order ----
+//
+// One screen value serves both rows of a pair. With acc = m_c - 2 x_q . x_c
+// (m_c = |x_c|^2 (1 - c), the accumulator's initial value), the row test
+// acc <= Tn_q collects c for q. With the roles swapped the pass would compare
+// acc' = m_q - 2 x_q . x_c against Tn_c; acc - m_c + m_q is that value up to
+// the same rounding bound the slack c already covers (the bound is symmetric
+// in the two rows), so the column test acc - W_c <= L_q collects q for c, with
+// W_c = Tn_c + m_c + 8u (|Tn_c| + 2 |x_c|^2 + T_c) rounded up (computed in
+// FP64 once per row) and L_q = 4u |x_q|^2 - m_q: the margins cover the
+// rounding of Tn_c and of the two extra operations (|acc| <= 2|x_c|^2 +
+// |x_q|^2 + c S). So each unordered pair of tiles is computed once: block B
+// (query rows [128 B, 128 B + 128), positions = local rows) takes candidate
+// super-tiles B + o (mod NB) for o in [0, H_B], H_B = (NB - 1) / 2, plus
+// o = NB / 2 when NB is even and B < NB / 2 - every unordered pair of
+// distinct super-tiles exactly once; blockIdx.y splits the offsets into S
+// slices. o = 0, its own super-tile, runs the row test only (that covers
+// every ordered pair in it).
+//
+// A row has 2S buffer slices. Row entries go to slice y (only this wave
+// writes it; the count stays in registers, as in the plain collect pass).
+// Column entries are kept per candidate row as (c, mask of this wave's 32
+// queries) words, appended with no atomics to a chunk of a shared pool that
+// this wave owns (one atomic per kChunkWords words takes a new chunk; its
+// header records the wave's query tile and slice); scatter_pairs_kernel
+// expands them into slice S + y of row c afterwards (one atomic per word).
+// Words that find the pool exhausted flag their rows, which then count as
+// overflowed. The per-(tile, half) maximum of W gives the coarse column test
+// min(acc) - Wmax <= L_q (rounding is monotone, so it misses nothing), two
+// instructions per tile.
+constexpr int kChunkWords = 256;
+
+template <class Op, int DP>
+__global__ void __launch_bounds__(64 * kTilesPerBlock) __attribute__((amdgpu_waves_per_eu(DP >= 128 ? 2 : 4, 8)))
+mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2* __restrict__ wmax,
+                        uint64_t* __restrict__ pool, int* __restrict__ chunk_region,
+                        int* __restrict__ chunk_count, int* __restrict__ next_chunk, int max_chunks,
+                        int* __restrict__ col_over) {
+    using Frag = typename Op::template Frag<DP>;
+    __shared__ __attribute__((aligned(16))) unsigned char smem[2 * ChunkShape<Op, DP>::kSymBytes];
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int j = lane & 31;
+    const int h = lane >> 5;
+    const int nb = static_cast<int>(gridDim.x);
+    const int B = static_cast<int>(blockIdx.x);
+    const int qtile = B * kTilesPerBlock + wave;
+    const int q = qtile * 32 + j;  // position == local row
+    const bool valid = q < p.n;
+    const int qi = p.qlist[valid ? q : 0];
+    const int ns = static_cast<int>(gridDim.y);
+
+    if (*p.guard) {
+        if (valid && h == 0 && blockIdx.y == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+        return;
+    }
+
+    Frag bq;
+    Op::template load_b<DP>(p.xc + static_cast<size_t>(qi) * DP, h, bq);
+    const float nq = p.norms[qi];
+    const float c = p.c;
+    const float t = p.tseed[qi];
+    // A heavy row collects nothing for itself (the list pass serves it) but
+    // still serves as a candidate of the other rows.
+    const float Tn = valid && !p.heavy[q] ? (__builtin_fmaf(t, 1.0f + c, c * nq) - nq) + 4.0f * kU * (t + nq)
+                                          : -__uint_as_float(kInfBits);
+    const float Lq = valid ? 4.0f * kU * nq - p.madj[q] : __uint_as_float(0x7FC00000u);
+    const int h_b = (nb - 1) / 2 + ((nb % 2 == 0 && B < nb / 2) ? 1 : 0);
+    const long long span = h_b + 1;  // offsets 0..h_b
+    const int o0 = static_cast<int>(span * blockIdx.y / ns);
+    const int o1 = static_cast<int>(span * (blockIdx.y + 1) / ns);
+    const int wrap = kTilesPerBlock * nb;
+    const int v_col = kTilesPerBlock * (B + 1);  // tiles of other super-tiles from here
+    const int cap = p.cap;
+    const int nslot = 2 * ns;
+    const size_t slot = static_cast<size_t>(valid ? q : 0) * nslot + blockIdx.y;
+    uint32_t* qbuf = p.cbuf + slot * cap;
+    const int region = qtile * ns + static_cast<int>(blockIdx.y);
+    int cnt = 0;                // row entries (the same in both lanes of a query)
+    int chunk = -1;             // this wave's pool chunk (the same in every lane)
+    int fill = kChunkWords;     // words used in it
+    auto room_for_two = [&]() {  // called by the whole wave
+        if (fill + 2 <= kChunkWords) return;
+        if (lane == 0 && chunk >= 0 && chunk < max_chunks) chunk_count[chunk] = fill;
+        int id = 0;
+        if (lane == 0) id = atomicAdd(next_chunk, 1);
+        chunk = __shfl(id, 0, 64);
+        fill = 0;
+        if (lane == 0 && chunk < max_chunks) chunk_region[chunk] = region;
+    };
+
+    auto visit = [&](const floatx16& acc, int v, float wm, const float4* wrow) {
+        float lo = acc[0];
+#pragma unroll
+        for (int r = 1; r < 16; ++r) lo = __builtin_fminf(lo, acc[r]);  // NaN rows drop out
+        const bool other_tile = v >= v_col;  // the same in the whole wave
+        const bool rowp = lo <= Tn;
+        const bool colp = other_tile && lo - wm <= Lq;
+        if (!__any(rowp || colp)) return;
+        const int pt = v >= wrap ? v - wrap : v;
+        const uint32_t c0 = static_cast<uint32_t>(pt * 32 + 4 * h);
+        if (__any(rowp)) {
+            uint32_t mask = 0;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) mask |= (acc[r] <= Tn ? 1u : 0u) << r;
+            const int mine = __popc(mask);
+            const int other = __shfl_xor(mine, 32, 64);
+            int pos = cnt + (h ? other : 0);
+            while (mask) {
+                const int r = __ffs(mask) - 1;
+                mask &= mask - 1;
+                if (pos < cap) qbuf[pos] = c0 + static_cast<uint32_t>(8 * (r / 4) + (r % 4));
+                ++pos;
+            }
+            cnt += mine + other;
+        }
+        if (other_tile && __any(colp)) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const float4 w = wrow[2 * b + h];
+                const float wv[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const uint64_t bal = __ballot(acc[4 * b + i] - wv[i] <= Lq);
+                    if (bal == 0) continue;
+                    room_for_two();
+                    // lane 0 writes half 0's word (row 8b + i), lane 32 half 1's (row 8b + 4 + i)
+                    const uint32_t lo = static_cast<uint32_t>(bal);
+                    const uint32_t hi = static_cast<uint32_t>(bal >> 32);
+                    const uint32_t m = h ? hi : lo;
+                    if (j == 0 && m != 0) {
+                        const int pos = fill + (h && lo != 0 ? 1 : 0);
+                        const uint32_t crow = c0 + static_cast<uint32_t>(8 * b + i);
+                        if (chunk < max_chunks) {
+                            pool[static_cast<size_t>(chunk) * kChunkWords + pos] =
+                                (static_cast<uint64_t>(crow) << 32) | m;
+                        } else {
+                            atomicOr(col_over + crow, 1);
+                        }
+                    }
+                    fill += (lo != 0) + (hi != 0);
+                }
+            }
+        }
+    };
+    stream_range<Op, DP, true>(p, bq, j, h, smem, kTilesPerBlock * (B + o0), kTilesPerBlock * (B + o1),
+                               wrap, wcol, wmax, visit);
+    if (valid && h == 0) p.ccount[slot] = cnt;
+    if (lane == 0 && chunk >= 0 && chunk < max_chunks) chunk_count[chunk] = fill;
+}
+
+// Column words of each pool chunk (written by query tile t, slice y) into
+// slice S + y of their rows: one atomic per word reserves its popcount, then
+// the queries t * 32 + bit are written. Rows whose words found the pool
+// exhausted are marked overflowed (their first column slice is set past cap).
+__global__ void __launch_bounds__(256)
+scatter_pairs_kernel(const uint64_t* __restrict__ pool, const int* __restrict__ chunk_region,
+                     const int* __restrict__ chunk_count, const int* __restrict__ next_chunk,
+                     int max_chunks, int ns, int cap, const int* __restrict__ guard,
+                     uint32_t* __restrict__ cbuf, int* __restrict__ ccount) {
+    if (*guard) return;
+    const int chunks = min(*next_chunk, max_chunks);
+    for (int ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
+        const int count = chunk_count[ch];
+        const int r = chunk_region[ch];
+        const uint64_t* src = pool + static_cast<size_t>(ch) * kChunkWords;
+        const uint32_t q0 = static_cast<uint32_t>(r / ns) * 32;
+        const int y = r % ns;
+        for (int e = threadIdx.x; e < count; e += 256) {
+            const uint64_t w = src[e];
+            uint32_t m = static_cast<uint32_t>(w);
+            const size_t slot = static_cast<size_t>(w >> 32) * (2 * ns) + ns + y;
+            int pos = atomicAdd(ccount + slot, __popc(m));
+            uint32_t* dst = cbuf + slot * cap;
+            while (m) {
+                const int bit = __ffs(m) - 1;
+                m &= m - 1;
+                if (pos < cap) dst[pos] = q0 + static_cast<uint32_t>(bit);
+                ++pos;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+mark_overflow_kernel(const int* __restrict__ col_over, int n, int ns, int cap, int* __restrict__ ccount) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n && col_over[i]) ccount[static_cast<size_t>(i) * (2 * ns) + ns] = cap + 1;
+}
+
+// Rows whose candidate set would be huge (outliers in sparse space, whose
+// seed threshold covers much of the data) skip the collect passes: the
+// number of m sample rows (every (n/m)-th position, so spread over all cells)
+// within the row's seed T estimates its candidate count; above kHeavyCount
+// the row is queued for the seeded list pass while the queue has room, and
+// flagged. In the symmetric pass a flagged row collects nothing and is never
+// a column row, so such rows cannot flood the column buffers. The estimate
+// only steers work; results do not depend on it.
+constexpr int kHeavyCount = 16384;
+constexpr int kHeavyChunk = 64;  // sample rows staged in LDS at a time
+
+template <int DP>
+__global__ void __launch_bounds__(256)
+heavy_rows_kernel(const float* __restrict__ xs_ord, const int* __restrict__ order,
+                  const float* __restrict__ tseed, int n, int m, int* __restrict__ heavy,
+                  int* __restrict__ list_rows, int* __restrict__ list_count, int list_cap) {
+    __shared__ __attribute__((aligned(16))) float s_smp[kHeavyChunk * DP];
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = q < n;
+    float xq[DP];
+    const float* row = xs_ord + static_cast<size_t>(valid ? q : 0) * DP;
+#pragma unroll
+    for (int t = 0; t < DP; ++t) xq[t] = row[t];
+    const float T = tseed[order[valid ? q : 0]];
+    int hits = 0;
+    for (int s0 = 0; s0 < m; s0 += kHeavyChunk) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < kHeavyChunk * DP; e += 256) {
+            const int sj = s0 + e / DP;
+            const size_t pos = static_cast<size_t>(static_cast<long long>(sj) * n / m);
+            s_smp[e] = sj < m ? xs_ord[pos * DP + e % DP] : __uint_as_float(0x7FC00000u);
+        }
+        __syncthreads();
+#pragma unroll 2
+        for (int r = 0; r < kHeavyChunk; ++r) {
+            float e = 0.0f;
+#pragma unroll
+            for (int t = 0; t < DP; ++t) {
+                const float df = xq[t] - s_smp[r * DP + t];
+                e = __builtin_fmaf(df, df, e);
+            }
+            hits += e <= T;
+        }
+    }
+    if (!valid) return;
+    int flag = 0;
+    if (static_cast<long long>(hits) * n > static_cast<long long>(kHeavyCount) * m) {
+        const int at = atomicAdd(list_count, 1);
+        if (at < list_cap) {
+            list_rows[at] = order[q];
+            flag = 1;
+        }
+    }
+    heavy[q] = flag;
+}
+
+// Column thresholds of the symmetric pass, per candidate position (NaN past n):
+// W = Tn + m + 8u (|Tn| + 2 |x|^2 + T), in FP64, rounded up to a float (-inf
+// for a heavy row); and per (tile, half) the maximum over the half's 16 rows
+// (8b + 4h + i).
+__global__ void __launch_bounds__(256)
+column_thresholds_kernel(const float* __restrict__ tseed, const float* __restrict__ norms,
+                         const float* __restrict__ madj, const int* __restrict__ cand,
+                         const int* __restrict__ heavy, int n, int padded_rows, float c,
+                         float* __restrict__ wcol) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= padded_rows) return;
+    if (i >= n) {
+        wcol[i] = __uint_as_float(0x7FC00000u);
+        return;
+    }
+    if (heavy[i]) {
+        wcol[i] = -__uint_as_float(kInfBits);
+        return;
+    }
+    const int o = cand[i];
+    const double t = tseed[o];
+    const double nc = norms[o];
+    const double u = 5.9604644775390625e-8;
+    const double cd = c;
+    const double tn = t * (1.0 + cd) + cd * nc - nc + 4.0 * u * (t + nc);
+    const double w = tn + static_cast<double>(madj[i]) + 8.0 * u * (fabs(tn) + 2.0 * nc + t);
+    const float f = static_cast<float>(w);
+    wcol[i] = static_cast<double>(f) < w ? __uint_as_float(__float_as_uint(f) + 1u) : f;
+}
+
+__global__ void __launch_bounds__(256)
+column_maxima_kernel(const float* __restrict__ wcol, int tiles, float2* __restrict__ wmax) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 2 * tiles) return;
+    const int tile = e >> 1;
+    const int h = e & 1;
+    float m = __uint_as_float(0x7FC00000u);
+    for (int b = 0; b < 4; ++b)
+        for (int i = 0; i < 4; ++i) m = __builtin_fmaxf(m, wcol[tile * 32 + 8 * b + 4 * h + i]);
+    reinterpret_cast<float*>(wmax)[e] = m;
+}
+
 // Collected entries per query the wave select holds; larger sets go to the
 // block select, which streams any number through LDS.
 constexpr int kSelectMax = 1024;
 constexpr int kBigChunk = 2048;  // words sorted per step of the block select
 
-__device__ __forceinline__ int lanes_below(uint64_t mask) {
-    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
-                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
-}
 
 // (key, original index) word of candidate position `pos` for query qi.
 template <int DP>
@@ -810,6 +1160,7 @@ collect_select_kernel(MfmaArgs p) {
     const int local = blockIdx.x * 4 + wave;
     if (local >= p.rows) return;  // whole waves; no block barrier below
     if (*p.guard) return;         // the sweep queued every row for the radix path
+    if (p.heavy && p.heavy[local]) return;  // already queued for the list pass
     const int qi = query_of(p, local);
     const int nb = p.slices;      // <= 32
     const size_t base = static_cast<size_t>(local) * nb;
@@ -1195,6 +1546,42 @@ void launch_collect(const MfmaArgs& a, hipStream_t stream) {
     }
 }
 
+// Buffers of the symmetric first pass besides the row slices.
+struct SymBuffers {
+    float* wcol;
+    float2* wmax;
+    uint64_t* pool;      // column words, in chunks of kChunkWords
+    int* chunk_region;   // per chunk: query tile * S + slice of the wave that owns it
+    int* chunk_count;    // per chunk: words written
+    int* next_chunk;     // chunks handed out
+    int max_chunks;
+    int* col_over;
+    int slices;  // S: offset slices (the rows have 2S buffer slices)
+};
+
+// Symmetric first pass of a full run, the column pairs into their rows, then
+// the selects.
+template <class Op, int DP>
+void launch_sym(const MfmaArgs& a, const SymBuffers& sb, hipStream_t stream) {
+    const int nb = (a.n + 32 * kTilesPerBlock - 1) / (32 * kTilesPerBlock);
+    {
+        ProfileScope prof("mfma_collect", stream);
+        mfma_sym_collect_kernel<Op, DP><<<dim3(nb, sb.slices), 64 * kTilesPerBlock, 0, stream>>>(
+            a, sb.wcol, sb.wmax, sb.pool, sb.chunk_region, sb.chunk_count, sb.next_chunk, sb.max_chunks,
+            sb.col_over);
+    }
+    {
+        ProfileScope prof("scatter_pairs", stream);
+        scatter_pairs_kernel<<<4096, 256, 0, stream>>>(sb.pool, sb.chunk_region, sb.chunk_count,
+                                                      sb.next_chunk, sb.max_chunks, sb.slices, a.cap,
+                                                      a.guard, a.cbuf, a.ccount);
+        mark_overflow_kernel<<<(a.n + 255) / 256, 256, 0, stream>>>(sb.col_over, a.n, sb.slices, a.cap, a.ccount);
+    }
+    ProfileScope prof("collect_select", stream);
+    collect_select_kernel<DP><<<(a.rows + 3) / 4, 256, 0, stream>>>(a);
+    big_select_kernel<DP><<<std::min(a.rows, 1024), 256, 0, stream>>>(a, a.big_rows, a.big_count);
+}
+
 template <class Op, int DP>
 bool dispatch_k(const MfmaArgs* a, int k, hipStream_t stream) {
     if (a && a->collect) {
@@ -1292,12 +1679,23 @@ int collect_cap(int) { return kSelectMax / 2; }
 // sets of many thousands, e.g. outliers in high dimension, whose distances
 // concentrate) go to the seeded list sweep over as many slices (kListRows of
 // them), which keeps no candidate buffer at all.
+// Pool chunks of the symmetric pass's column words: 256 words (one chunk)
+// per row, at least 4096 chunks.
+int sym_chunks(int n_points) { return std::max(n_points, 4096); }
+
+// Operand rows are padded to whole blocks of query tiles (the symmetric pass
+// pairs blocks; the other kernels ignore the extra rows).
+size_t padded_positions(int n_points) {
+    const size_t b = 32 * kTilesPerBlock;
+    return (static_cast<size_t>(n_points) + b - 1) / b * b;
+}
+
 // Screen operand rows (form layout) and adjusted norms, in candidate order
 // (cand: position -> original index; null = natural order).
 void prepare_screen(const float* centred, const float* norms, const int* cand, int n_points, int dp,
                     bool bf16, float c, void* xa, float* madj, hipStream_t stream) {
     ProfileScope prof("screen_prep", stream);
-    const int padded_rows = (n_points + 31) / 32 * 32;
+    const int padded_rows = static_cast<int>(padded_positions(n_points));
     const size_t padded_cells = static_cast<size_t>(padded_rows) * dp;
     const unsigned grid = static_cast<unsigned>((padded_cells + 255) / 256);
     if (bf16) {
@@ -1321,8 +1719,28 @@ int second_slices(int n_points) {
     return std::max(1, std::min(32, tiles / 8));
 }
 
+// Symmetric first pass (a full run with the candidate order): 2S slices of
+// kSymCap entries per row (S row slices, S column slices); a row with a full
+// slice goes to the second pass.
+// CULOF_SYM=0 turns it off (the same results; for measurement).
+constexpr int kSymCap = 1024;
+bool sym_enabled() {
+    const char* env = std::getenv("CULOF_SYM");
+    return !(env && env[0] == '0');
+}
+// Offsets of the half ring split so the pass has about 16 waves per SIMD (a
+// short tail when the last round of waves is partly filled), with at least 8
+// super-tiles per slice and at most 16 slices (the selects take 32 buffer
+// slices per row).
+int sym_slices(int n_points) {
+    const int nb = (n_points + 32 * kTilesPerBlock - 1) / (32 * kTilesPerBlock);
+    const int by_waves = (16384 + kTilesPerBlock * nb - 1) / (kTilesPerBlock * nb);
+    const int by_len = std::max(1, (nb / 2 + 1) / 8);
+    return std::max(1, std::min(std::min(by_waves, by_len), 16));
+}
+
 size_t mfma_scratch_bytes(int n_points, int dp, int form, int rows, int k, bool collect) {
-    const size_t padded = static_cast<size_t>((n_points + 31) / 32) * 32;
+    const size_t padded = padded_positions(n_points);
     const size_t rows_bytes = form == kFormBF16 ? 2 * sizeof(uint16_t) : sizeof(float);
     const size_t operand = padded * dp * rows_bytes;
     const size_t madj = sizeof(float) * padded;
@@ -1331,8 +1749,13 @@ size_t mfma_scratch_bytes(int n_points, int dp, int form, int rows, int k, bool 
     if (collect) {
         const size_t slots = static_cast<size_t>(rows) * s;
         const size_t second = static_cast<size_t>(std::min(rows, kSecondRows)) * second_slices(n_points);
-        lists = slots * kSelectMax * sizeof(uint32_t) + slots * sizeof(int) +
-                static_cast<size_t>(rows) * 2 * sizeof(int) + 4 * sizeof(int) +
+        const size_t ss = static_cast<size_t>(sym_slices(n_points));
+        const size_t sym = static_cast<size_t>(rows) * (2 * ss * (kSymCap + 1) + 2) * sizeof(uint32_t) +
+                           padded * sizeof(float) + padded / 32 * sizeof(float2) +
+                           static_cast<size_t>(sym_chunks(n_points)) * (kChunkWords * sizeof(uint64_t) + 2 * sizeof(int)) +
+                           2048;
+        const size_t first = std::max(slots * (kSelectMax + 1) * sizeof(uint32_t), sym) + 256;
+        lists = first + static_cast<size_t>(rows) * 2 * sizeof(int) + 4 * sizeof(int) +
                 second * kSecondCap * sizeof(uint32_t) + second * sizeof(int) +
                 static_cast<size_t>(std::min(rows, kListRows)) * (sizeof(int) +
                     sizeof(uint64_t) * second_slices(n_points) * k) + 16 * 256;
@@ -1353,7 +1776,7 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
     if (ord && ord->dp != dp) throw std::invalid_argument("mfma_knn: order built for another stride");
     const bool bf16 = form == kFormBF16;
     const float c = bf16 ? ScreenBF16::slack(dp) : ScreenF32::slack(dp);
-    const int padded_rows = (n_points + 31) / 32 * 32;
+    const int padded_rows = static_cast<int>(padded_positions(n_points));
     const size_t padded_cells = static_cast<size_t>(padded_rows) * dp;
     unsigned char* base = static_cast<unsigned char*>(scratch);
     auto take = [&](size_t bytes) {
@@ -1373,10 +1796,37 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
                   kdist2, fallback_rows, fallback_count};
     MfmaArgs second = args;
     MfmaArgs list = args;
+    // A full run in candidate order computes each pair of tiles once.
+    const bool sym = collect && ord && qlist == ord->order && row0 == 0 && rows == n_points && sym_enabled();
+    SymBuffers sb{};
+    int* heavy = nullptr;
     if (collect) {
-        const size_t slots = static_cast<size_t>(rows) * slices;
-        args.cbuf = static_cast<uint32_t*>(take(slots * kSelectMax * sizeof(uint32_t)));
-        args.ccount = static_cast<int*>(take(slots * sizeof(int)));
+        if (sym) {
+            sb.slices = sym_slices(n_points);
+            args.slices = 2 * sb.slices;
+            args.cap = kSymCap;
+            const size_t slots = static_cast<size_t>(rows) * args.slices;
+            args.cbuf = static_cast<uint32_t*>(take(slots * kSymCap * sizeof(uint32_t)));
+            args.ccount = static_cast<int*>(take(slots * sizeof(int)));
+            heavy = static_cast<int*>(take(static_cast<size_t>(rows) * sizeof(int)));
+            args.heavy = heavy;
+            sb.wcol = static_cast<float*>(take(sizeof(float) * padded_rows));
+            sb.wmax = static_cast<float2*>(take(sizeof(float2) * (padded_rows / 32)));
+            sb.max_chunks = sym_chunks(n_points);
+            const size_t chunks = static_cast<size_t>(sb.max_chunks);
+            sb.pool = static_cast<uint64_t*>(take(chunks * kChunkWords * sizeof(uint64_t)));
+            sb.chunk_region = static_cast<int*>(take(chunks * sizeof(int)));
+            sb.chunk_count = static_cast<int*>(take(chunks * sizeof(int)));
+            sb.next_chunk = static_cast<int*>(take(sizeof(int)));
+            sb.col_over = static_cast<int*>(take(static_cast<size_t>(rows) * sizeof(int)));
+            CULOF_CHECK(hipMemsetAsync(args.ccount, 0, slots * sizeof(int), stream));
+            CULOF_CHECK(hipMemsetAsync(sb.next_chunk, 0, sizeof(int), stream));
+            CULOF_CHECK(hipMemsetAsync(sb.col_over, 0, static_cast<size_t>(rows) * sizeof(int), stream));
+        } else {
+            const size_t slots = static_cast<size_t>(rows) * slices;
+            args.cbuf = static_cast<uint32_t*>(take(slots * kSelectMax * sizeof(uint32_t)));
+            args.ccount = static_cast<int*>(take(slots * sizeof(int)));
+        }
         args.big_rows = static_cast<int*>(take(static_cast<size_t>(rows) * sizeof(int)));
         args.ov_cap = std::min(rows, kSecondRows);
         args.ov_rows = static_cast<int*>(take(static_cast<size_t>(args.ov_cap) * sizeof(int)));
@@ -1397,6 +1847,7 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
         second.ccount = static_cast<int*>(take(slots2 * sizeof(int)));
         second.big_rows = nullptr;
         second.big_count = nullptr;
+        second.heavy = nullptr;
         // Its overflow queue feeds the list pass.
         second.ov_cap = std::min(rows, kListRows);
         second.ov_rows = static_cast<int*>(take(static_cast<size_t>(second.ov_cap) * sizeof(int)));
@@ -1413,6 +1864,7 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
                                        : nullptr;
         list.cbuf = nullptr;
         list.ccount = nullptr;
+        list.heavy = nullptr;
         // First-pass rows that find the second pass's queue full go there too.
         args.list_rows = second.ov_rows;
         args.list_count = second.ov_count;
@@ -1428,7 +1880,39 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
         if (!ok) throw std::invalid_argument("mfma_knn: unsupported (n_dims, k, form)");
         CULOF_CHECK(hipGetLastError());
     };
-    run(args);
+    if (sym) {
+        {
+            // Rows with huge candidate sets go to the list pass up front.
+            ProfileScope prof("heavy_rows", stream);
+            const int m = std::min(n_points, dp <= 32 ? 512 : 256);
+            auto go = [&](auto D) {
+                heavy_rows_kernel<decltype(D)::value><<<(n_points + 255) / 256, 256, 0, stream>>>(
+                    ord->xs_ord, ord->order, tseed, n_points, m, heavy, second.ov_rows, second.ov_count,
+                    second.ov_cap);
+                return true;
+            };
+            for_dp<ScreenF32>(dp, go);
+            CULOF_CHECK(hipGetLastError());
+        }
+        column_thresholds_kernel<<<(padded_rows + 255) / 256, 256, 0, stream>>>(
+            tseed, norms, madj, ord->order, heavy, n_points, padded_rows, c, sb.wcol);
+        column_maxima_kernel<<<(2 * (padded_rows / 32) + 255) / 256, 256, 0, stream>>>(
+            sb.wcol, padded_rows / 32, sb.wmax);
+        CULOF_CHECK(hipGetLastError());
+        auto go_bf16 = [&](auto D) {
+            launch_sym<ScreenBF16, decltype(D)::value>(args, sb, stream);
+            return true;
+        };
+        auto go_f32 = [&](auto D) {
+            launch_sym<ScreenF32, decltype(D)::value>(args, sb, stream);
+            return true;
+        };
+        const bool ok = k <= 64 && (bf16 ? for_dp<ScreenBF16>(dp, go_bf16) : for_dp<ScreenF32>(dp, go_f32));
+        if (!ok) throw std::invalid_argument("mfma_knn: unsupported (n_dims, k, form)");
+        CULOF_CHECK(hipGetLastError());
+    } else {
+        run(args);
+    }
     if (collect) {
         run(second);
         run(list);
@@ -1451,7 +1935,7 @@ int tie_band_capacity(int n_points, int n_dims, int stride, int form, int rows) 
 
 size_t tie_band_scratch_bytes(int n_points, int n_dims, int form, int cap) {
     const int dp = mfma_padded_dims(n_dims, form);
-    const size_t padded = static_cast<size_t>((n_points + 31) / 32) * 32;
+    const size_t padded = padded_positions(n_points);
     const size_t slots = static_cast<size_t>(cap) * second_slices(n_points);
     return padded * dp * 4 + 256 + padded * sizeof(float) + 256 + slots * kBandCap * sizeof(uint32_t) +
            256 + slots * sizeof(int) + 256;
@@ -1463,7 +1947,7 @@ void tie_band_select(const float* keyrows, int stride, int n_points, int k, cons
     const int dp = stride;
     const bool bf16 = band.form == kFormBF16;
     const float c = bf16 ? ScreenBF16::slack(dp) : ScreenF32::slack(dp);
-    const int padded_rows = (n_points + 31) / 32 * 32;
+    const int padded_rows = static_cast<int>(padded_positions(n_points));
     unsigned char* base = static_cast<unsigned char*>(band.scratch);
     auto take = [&](size_t bytes) {
         void* r = base;
@@ -1481,15 +1965,15 @@ void tie_band_select(const float* keyrows, int stride, int n_points, int k, cons
                band.guard, n_points, k, c, 0, band.cap, slices, nullptr, 1, band.count, cbuf, ccount,
                kBandCap, nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr, 0, nbr_idx, nbr_d2,
                kdist2, brute_rows, brute_count};
-    auto go = [&](auto D) {
-        if (bf16) {
-            launch_band<ScreenBF16, decltype(D)::value>(a, band.bound, brute_rows, brute_count, stream);
-        } else {
-            launch_band<ScreenF32, decltype(D)::value>(a, band.bound, brute_rows, brute_count, stream);
-        }
+    auto go_bf16 = [&](auto D) {
+        launch_band<ScreenBF16, decltype(D)::value>(a, band.bound, brute_rows, brute_count, stream);
         return true;
     };
-    const bool ok = bf16 ? for_dp<ScreenBF16>(dp, go) : for_dp<ScreenF32>(dp, go);
+    auto go_f32 = [&](auto D) {
+        launch_band<ScreenF32, decltype(D)::value>(a, band.bound, brute_rows, brute_count, stream);
+        return true;
+    };
+    const bool ok = bf16 ? for_dp<ScreenBF16>(dp, go_bf16) : for_dp<ScreenF32>(dp, go_f32);
     if (!ok) throw std::invalid_argument("tie band: unsupported (n_dims, form)");
     CULOF_CHECK(hipGetLastError());
 }

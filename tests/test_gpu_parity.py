@@ -239,6 +239,31 @@ def test_order_switch_gives_same_bits(monkeypatch):
         np.testing.assert_array_equal(default[key], plain[key], err_msg=key)
 
 
+@pytest.mark.parametrize("n,d,k,form", [(20_000, 8, 20, 0), (16_384, 16, 10, 1), (5_000, 64, 31, 0),
+                                         (9_100, 32, 15, 1), (300, 4, 7, 0)])
+def test_symmetric_pass_switch_gives_same_bits(monkeypatch, n, d, k, form):
+    """The symmetric first pass (each pair of tiles computed once, the screen
+    value tested against both rows' thresholds) and CULOF_SYM=0 (every query
+    row against every candidate tile) agree bit for bit, on clustered data
+    with far outliers (large norms next to small thresholds), for odd and even
+    block counts; sampled rows also match the oracle."""
+    from _gpu import knn
+
+    rng = np.random.default_rng(n + d)
+    X = make_data(n, d, n % 97, max(1, n // 200))
+    far = rng.choice(n, max(1, n // 100), replace=False)
+    X[far] = X[far] * 40.0 + 25.0
+    sym = knn(X, k, form=form)
+    monkeypatch.setenv("CULOF_SYM", "0")
+    plain = knn(X, k, form=form)
+    for a, b, name in zip(sym, plain, ("idx", "d2", "kdist2")):
+        np.testing.assert_array_equal(a, b, err_msg=name)
+    for r in np.r_[far[:3], rng.choice(n, 5, replace=False)]:
+        oi, od, ok = oracle.knn_rows(X, k, int(r), 1)
+        np.testing.assert_array_equal(sym[0][:, r], oi[0], err_msg=f"row {r}")
+        _assert_same_bits(sym[2][r], ok[0], f"row {r} kdist2")
+
+
 def test_every_fallback_tier():
     """10,000 copies of one point among 150,000: their collected sets overflow
     the first pass, the second pass and the list queue, so the rows run
