@@ -294,7 +294,7 @@ void knn_rows(const Prepared& p, int n, int d, int k, int form, int row0, int ro
                               scratch.get<unsigned char>(order_scratch_bytes(n, p.stride)), s);
             tseed = scratch.get<float>(static_cast<size_t>(n));
             if (row0 == 0 && rows == n) qorder = ord.order;
-            seed_thresholds(ord, n, k, tseed, s);
+            seed_thresholds(ord, n, padded_dims(d), k, tseed, s);
         }
     }
     for (int r0 = 0; r0 < rows; r0 += tile) {

@@ -126,8 +126,9 @@ bool order_supported(int dp);
 size_t order_scratch_bytes(int n, int dp);
 CandidateOrder build_order(const float* xs, int n, int dp, void* scratch, hipStream_t s);
 // tseed[i] for every point i = an upper bound of its k-th key (the k-th among
-// a window of candidates near it in the order).
-void seed_thresholds(const CandidateOrder& o, int n, int k, float* tseed, hipStream_t s);
+// a window of candidates near it in the order). key_dp: the padded feature
+// count of the keys (<= o.dp; the rest of each row is zero padding).
+void seed_thresholds(const CandidateOrder& o, int n, int key_dp, int k, float* tseed, hipStream_t s);
 
 // ---- ties.hip: FP64 tie-break over a (k+1)-neighbour result ----
 // Relative band (1 + g') / (1 - g') of FP32 keys that FP64 may reorder.

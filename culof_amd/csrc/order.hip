@@ -222,7 +222,9 @@ gather_rows_kernel(const float* __restrict__ src, int dp, const int* __restrict_
 // 64 RPL rows (lane l keys rows l, l + 64, ... of the chunk against the
 // wave's QPW queries; the next chunk is fetched into registers meanwhile),
 // then each query's k-th smallest key by a bitwise search on the key bits.
-template <int DP, int W, int QPW, int RPL>
+// DP: the features keyed (a multiple of 4); STRIDE: the row stride of xs_ord
+// (>= DP; the features past DP are zero padding, which adds exact zeros).
+template <int DP, int STRIDE, int W, int QPW, int RPL>
 __global__ void __launch_bounds__(kBlock)
 seed_kernel(const float* __restrict__ xs_ord, const int* __restrict__ order, int n, int k,
             float* __restrict__ tseed) {
@@ -245,7 +247,7 @@ seed_kernel(const float* __restrict__ xs_ord, const int* __restrict__ order, int
         const int v = e - t * (DP / 4);
         const int pos = p0 + t < n ? p0 + t : n - 1;
         reinterpret_cast<float4*>(&s_q[t][0])[v] =
-            reinterpret_cast<const float4*>(xs_ord + static_cast<size_t>(pos) * DP)[v];
+            reinterpret_cast<const float4*>(xs_ord + static_cast<size_t>(pos) * STRIDE)[v];
     }
     float4 nxt[kLoads];
     auto fetch = [&](int ch) {
@@ -255,7 +257,7 @@ seed_kernel(const float* __restrict__ xs_ord, const int* __restrict__ order, int
             const int r = e / (DP / 4);
             const int c = ch * kRows + r;
             nxt[u] = (e < kF4 && c < w)
-                         ? reinterpret_cast<const float4*>(xs_ord + static_cast<size_t>(c0 + c) * DP)[e - r * (DP / 4)]
+                         ? reinterpret_cast<const float4*>(xs_ord + static_cast<size_t>(c0 + c) * STRIDE)[e - r * (DP / 4)]
                          : make_float4(0.f, 0.f, 0.f, 0.f);
         }
     };
@@ -347,14 +349,14 @@ void launch_order(const float* xs, int n, int* cell1, int* hist1, int* offset1, 
     inverse_kernel<<<grid, kBlock, 0, s>>>(order, n, inv);
 }
 
-template <int DP>
+template <int DP, int STRIDE = DP>
 void launch_seed(const float* xs_ord, const int* order, int n, int k, float* tseed, hipStream_t s) {
     // Narrow rows afford a wider window (tighter seeds) for the same work.
     constexpr int W = (DP <= 8 || DP > 32) ? 1024 : 2048;
     constexpr int QPW = DP <= 32 ? 2 : 4;
     constexpr int RPL = DP <= 8 ? 8 : (DP <= 32 ? 2 : 1);  // about 16 KiB of rows per chunk
     const unsigned grid = static_cast<unsigned>((n + 4 * QPW - 1) / (4 * QPW));
-    seed_kernel<DP, W, QPW, RPL><<<grid, kBlock, 0, s>>>(xs_ord, order, n, k, tseed);
+    seed_kernel<DP, STRIDE, W, QPW, RPL><<<grid, kBlock, 0, s>>>(xs_ord, order, n, k, tseed);
 }
 
 }  // namespace
@@ -412,8 +414,17 @@ CandidateOrder build_order(const float* xs, int n, int dp, void* scratch, hipStr
     return o;
 }
 
-void seed_thresholds(const CandidateOrder& o, int n, int k, float* tseed, hipStream_t s) {
+void seed_thresholds(const CandidateOrder& o, int n, int key_dp, int k, float* tseed, hipStream_t s) {
     ProfileScope prof("seed", s);
+    if (o.dp == 16 && key_dp < 16) {  // narrow rows padded for the BF16 screen: key the real width
+        if (key_dp == 4) {
+            launch_seed<4, 16>(o.xs_ord, o.order, n, k, tseed, s);
+        } else {
+            launch_seed<8, 16>(o.xs_ord, o.order, n, k, tseed, s);
+        }
+        CULOF_CHECK(hipGetLastError());
+        return;
+    }
     switch (o.dp) {
         case 4: launch_seed<4>(o.xs_ord, o.order, n, k, tseed, s); break;
         case 8: launch_seed<8>(o.xs_ord, o.order, n, k, tseed, s); break;
