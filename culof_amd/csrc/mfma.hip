@@ -141,6 +141,24 @@ struct ScreenF32 {
         for (int m = 0; m < DP / 2; ++m) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.v[m], b.v[m], acc, 0, 0, 0);
         return acc;
     }
+    // The same products for two query fragments, the A values read from LDS
+    // step by step (one A fragment never sits whole in registers).
+    template <int DP>
+    __device__ static void run2(const Elem* half_row, const Frag<DP>& b0, const Frag<DP>& b1, floatx16& acc0,
+                                floatx16& acc1) {
+        static_assert(DP % 8 == 0, "float4 steps");
+        const float4* p = reinterpret_cast<const float4*>(half_row);
+#pragma unroll
+        for (int v = 0; v < DP / 8; ++v) {
+            const float4 t = p[v];
+            const float a[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b0.v[4 * v + i], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b1.v[4 * v + i], acc1, 0, 0, 0);
+            }
+        }
+    }
     __host__ __device__ static float slack(int dp) { return (2.0f * dp + 32.0f) * kU; }
 };
 
@@ -193,6 +211,24 @@ struct ScreenBF16 {
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.hi[m], b.hi[m], acc, 0, 0, 0);
         }
         return acc;
+    }
+    // The same products for two query fragments (each in the order of run),
+    // the A values read from LDS step by step.
+    template <int DP>
+    __device__ static void run2(const Elem* half_row, const Frag<DP>& b0, const Frag<DP>& b1, floatx16& acc0,
+                                floatx16& acc1) {
+        const bf16x8* p = reinterpret_cast<const bf16x8*>(half_row);
+#pragma unroll
+        for (int m = 0; m < DP / 16; ++m) {
+            const bf16x8 ahi = p[2 * m];
+            const bf16x8 alo = p[DP / 8 + 2 * m];
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, b0.hi[m], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, b1.hi[m], acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, b0.lo[m], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, b1.lo[m], acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, b0.hi[m], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, b1.hi[m], acc1, 0, 0, 0);
+        }
     }
     // 3.1 2^-16 for the split products, FP32 accumulation of 3 DP terms whose
     // magnitudes sum to at most 2 (N_q + N_i) (counted at 2 ulp per addition,
@@ -342,11 +378,12 @@ struct MfmaArgs {
 // column thresholds (8 float4s: rows 4i..4i+3). The operand arrays are padded
 // to whole tiles (zero rows, NaN norms and thresholds); tiles past t_end are
 // never visited. `stage` holds 2 * ChunkShape::kBytes (kSymBytes with kSym).
-template <class Op, int DP, bool kSym, class Visit>
-__device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename Op::template Frag<DP>& bq,
+template <class Op, int DP, bool kSym, int QT = 1, class Visit>
+__device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename Op::template Frag<DP>* bq,
                                              int j, int h, unsigned char* stage, int t_begin, int t_end,
                                              int wrap, const float* __restrict__ wcol,
                                              const float2* __restrict__ wmax, Visit&& visit) {
+    static_assert(QT == 1 || (QT == 2 && kSym), "two query tiles per wave: symmetric pass only");
     using Frag = typename Op::template Frag<DP>;
     using Elem = typename Op::Elem;
     using Shape = ChunkShape<Op, DP>;
@@ -432,9 +469,9 @@ __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename O
         return acc;
     };
     float wm[kSym ? CT : 1];  // kSym: this half's column maxima of the chunk's tiles
-    auto call = [&](const floatx16& acc, int buf, int tt, int v) {
+    auto call = [&](const floatx16& acc, int buf, int tt, int v, int t) {
         if constexpr (kSym) {
-            visit(acc, v, wm[tt], s_w + buf * kChunkM4 + tt * 8);
+            visit(acc, v, wm[tt], s_w + buf * kChunkM4 + tt * 8, t);
         } else {
             visit(acc, v);
         }
@@ -458,25 +495,42 @@ __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename O
                 wm[tt] = h ? x.y : x.x;
             }
         }
-        // Two tiles per step: two independent MFMA chains hide each other's
-        // result latency.
+        if constexpr (QT == 2) {
+            // One tile per step against two query fragments: two independent
+            // MFMA chains per A value read.
 #pragma unroll
-        for (int tt = 0; tt < CT; tt += 2) {
-            Frag f0, f1;
-            Op::template load_a<DP>(reinterpret_cast<const Elem*>(sa + (tt * 32 + j) * kPitchF4) +
-                                        Op::half_offset(DP, h), f0);
-            floatx16 acc0 = init_acc(sm, tt);
-            const bool two = tt + 1 < CT;
-            floatx16 acc1;
-            if (two) {
-                Op::template load_a<DP>(reinterpret_cast<const Elem*>(sa + ((tt + 1) * 32 + j) * kPitchF4) +
-                                            Op::half_offset(DP, h), f1);
-                acc1 = init_acc(sm, tt + 1);
+            for (int tt = 0; tt < CT; ++tt) {
+                floatx16 acc0 = init_acc(sm, tt);
+                floatx16 acc1 = acc0;
+                Op::template run2<DP>(reinterpret_cast<const Elem*>(sa + (tt * 32 + j) * kPitchF4) +
+                                          Op::half_offset(DP, h),
+                                      bq[0], bq[1], acc0, acc1);
+                if (t0 + tt < t_end) {
+                    call(acc0, buf, tt, t0 + tt, 0);
+                    call(acc1, buf, tt, t0 + tt, 1);
+                }
             }
-            acc0 = Op::template run<DP>(f0, bq, acc0);
-            if (two) acc1 = Op::template run<DP>(f1, bq, acc1);
-            if (t0 + tt < t_end) call(acc0, buf, tt, t0 + tt);
-            if (two && t0 + tt + 1 < t_end) call(acc1, buf, tt + 1, t0 + tt + 1);
+        } else {
+            // Two tiles per step: two independent MFMA chains hide each
+            // other's result latency.
+#pragma unroll
+            for (int tt = 0; tt < CT; tt += 2) {
+                Frag f0, f1;
+                Op::template load_a<DP>(reinterpret_cast<const Elem*>(sa + (tt * 32 + j) * kPitchF4) +
+                                            Op::half_offset(DP, h), f0);
+                floatx16 acc0 = init_acc(sm, tt);
+                const bool two = tt + 1 < CT;
+                floatx16 acc1;
+                if (two) {
+                    Op::template load_a<DP>(reinterpret_cast<const Elem*>(sa + ((tt + 1) * 32 + j) * kPitchF4) +
+                                                Op::half_offset(DP, h), f1);
+                    acc1 = init_acc(sm, tt + 1);
+                }
+                acc0 = Op::template run<DP>(f0, bq[0], acc0);
+                if (two) acc1 = Op::template run<DP>(f1, bq[0], acc1);
+                if (t0 + tt < t_end) call(acc0, buf, tt, t0 + tt, 0);
+                if (two && t0 + tt + 1 < t_end) call(acc1, buf, tt + 1, t0 + tt + 1, 0);
+            }
         }
         if (ch + 1 < n_chunks) stage_chunk(buf ^ 1);
         __syncthreads();
@@ -490,7 +544,7 @@ __device__ __forceinline__ void stream_tiles(const MfmaArgs& p, const typename O
     const int n_tiles = (p.n + 31) / 32;
     const int t_begin = static_cast<int>((static_cast<long long>(n_tiles) * blockIdx.y) / p.slices);
     const int t_end = static_cast<int>((static_cast<long long>(n_tiles) * (blockIdx.y + 1)) / p.slices);
-    stream_range<Op, DP, false>(p, bq, j, h, stage, t_begin, t_end, n_tiles, nullptr, nullptr, visit);
+    stream_range<Op, DP, false>(p, &bq, j, h, stage, t_begin, t_end, n_tiles, nullptr, nullptr, visit);
 }
 
 // Wide lists need the whole register file: let those instantiations run at
@@ -776,15 +830,26 @@ mfma_collect_kernel(MfmaArgs p) {
 // overflowed. The per-(tile, half) maximum of W gives the coarse column test
 // min(acc) - Wmax <= L_q (rounding is monotone, so it misses nothing), two
 // instructions per tile.
-constexpr int kChunkWords = 256;
+constexpr int kChunkWords = 128;
+
+// QT query tiles per wave (2 for the wide BF16 screen: each A value read from
+// LDS then feeds two MFMA chains, halving the operand reads per pair); a
+// block covers 4 QT query tiles and the ring is over blocks of 128 QT rows.
+template <class Op, int DP>
+constexpr int sym_query_tiles() {
+    return (std::is_same_v<Op, ScreenBF16> && DP >= 64) ? 2 : 1;
+}
 
 template <class Op, int DP>
-__global__ void __launch_bounds__(64 * kTilesPerBlock) __attribute__((amdgpu_waves_per_eu(DP >= 128 ? 2 : 4, 8)))
+__global__ void __launch_bounds__(64 * kTilesPerBlock)
+__attribute__((amdgpu_waves_per_eu(DP >= 128 || sym_query_tiles<Op, DP>() == 2 ? 2 : 4, 8)))
 mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2* __restrict__ wmax,
                         uint64_t* __restrict__ pool, int* __restrict__ chunk_region,
                         int* __restrict__ chunk_count, int* __restrict__ next_chunk, int max_chunks,
                         int* __restrict__ col_over) {
     using Frag = typename Op::template Frag<DP>;
+    constexpr int QT = sym_query_tiles<Op, DP>();
+    constexpr int kBlockTiles = kTilesPerBlock * QT;
     __shared__ __attribute__((aligned(16))) unsigned char smem[2 * ChunkShape<Op, DP>::kSymBytes];
 
     const int lane = threadIdx.x & 63;
@@ -793,75 +858,93 @@ mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2
     const int h = lane >> 5;
     const int nb = static_cast<int>(gridDim.x);
     const int B = static_cast<int>(blockIdx.x);
-    const int qtile = B * kTilesPerBlock + wave;
-    const int q = qtile * 32 + j;  // position == local row
-    const bool valid = q < p.n;
-    const int qi = p.qlist[valid ? q : 0];
     const int ns = static_cast<int>(gridDim.y);
+    const int qtile0 = (B * kTilesPerBlock + wave) * QT;
+
+    int q[QT];
+    bool valid[QT];
+    int qi[QT];
+#pragma unroll
+    for (int t = 0; t < QT; ++t) {
+        q[t] = (qtile0 + t) * 32 + j;  // position == local row
+        valid[t] = q[t] < p.n;
+        qi[t] = p.qlist[valid[t] ? q[t] : 0];
+    }
 
     if (*p.guard) {
-        if (valid && h == 0 && blockIdx.y == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+#pragma unroll
+        for (int t = 0; t < QT; ++t)
+            if (valid[t] && h == 0 && blockIdx.y == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi[t];
         return;
     }
 
-    Frag bq;
-    Op::template load_b<DP>(p.xc + static_cast<size_t>(qi) * DP, h, bq);
-    const float nq = p.norms[qi];
+    Frag bq[QT];
+    float Tn[QT], Lq[QT];
     const float c = p.c;
-    const float t = p.tseed[qi];
-    // A heavy row collects nothing for itself (the list pass serves it) but
-    // still serves as a candidate of the other rows.
-    const float Tn = valid && !p.heavy[q] ? (__builtin_fmaf(t, 1.0f + c, c * nq) - nq) + 4.0f * kU * (t + nq)
-                                          : -__uint_as_float(kInfBits);
-    const float Lq = valid ? 4.0f * kU * nq - p.madj[q] : __uint_as_float(0x7FC00000u);
+#pragma unroll
+    for (int t = 0; t < QT; ++t) {
+        Op::template load_b<DP>(p.xc + static_cast<size_t>(qi[t]) * DP, h, bq[t]);
+        const float nq = p.norms[qi[t]];
+        const float tq = p.tseed[qi[t]];
+        // A heavy row collects nothing for itself (the list pass serves it)
+        // but still serves as a candidate of the other rows.
+        Tn[t] = valid[t] && !p.heavy[q[t]] ? (__builtin_fmaf(tq, 1.0f + c, c * nq) - nq) + 4.0f * kU * (tq + nq)
+                                           : -__uint_as_float(kInfBits);
+        Lq[t] = valid[t] ? 4.0f * kU * nq - p.madj[q[t]] : __uint_as_float(0x7FC00000u);
+    }
     const int h_b = (nb - 1) / 2 + ((nb % 2 == 0 && B < nb / 2) ? 1 : 0);
     const long long span = h_b + 1;  // offsets 0..h_b
     const int o0 = static_cast<int>(span * blockIdx.y / ns);
     const int o1 = static_cast<int>(span * (blockIdx.y + 1) / ns);
-    const int wrap = kTilesPerBlock * nb;
-    const int v_col = kTilesPerBlock * (B + 1);  // tiles of other super-tiles from here
+    const int wrap = kBlockTiles * nb;
+    const int v_col = kBlockTiles * (B + 1);  // tiles of other blocks from here
     const int cap = p.cap;
     const int nslot = 2 * ns;
-    const size_t slot = static_cast<size_t>(valid ? q : 0) * nslot + blockIdx.y;
-    uint32_t* qbuf = p.cbuf + slot * cap;
-    const int region = qtile * ns + static_cast<int>(blockIdx.y);
-    int cnt = 0;                // row entries (the same in both lanes of a query)
-    int chunk = -1;             // this wave's pool chunk (the same in every lane)
-    int fill = kChunkWords;     // words used in it
-    auto room_for_two = [&]() {  // called by the whole wave
-        if (fill + 2 <= kChunkWords) return;
-        if (lane == 0 && chunk >= 0 && chunk < max_chunks) chunk_count[chunk] = fill;
+    uint32_t* qbuf[QT];
+    int cnt[QT];      // row entries (the same in both lanes of a query)
+    int chunk[QT];    // this wave's pool chunk per query tile (the same in every lane)
+    int fill[QT];     // words used in it
+#pragma unroll
+    for (int t = 0; t < QT; ++t) {
+        qbuf[t] = p.cbuf + (static_cast<size_t>(valid[t] ? q[t] : 0) * nslot + blockIdx.y) * cap;
+        cnt[t] = 0;
+        chunk[t] = -1;
+        fill[t] = kChunkWords;
+    }
+    auto room_for_two = [&](int t) {  // called by the whole wave
+        if (fill[t] + 2 <= kChunkWords) return;
+        if (lane == 0 && chunk[t] >= 0 && chunk[t] < max_chunks) chunk_count[chunk[t]] = fill[t];
         int id = 0;
         if (lane == 0) id = atomicAdd(next_chunk, 1);
-        chunk = __shfl(id, 0, 64);
-        fill = 0;
-        if (lane == 0 && chunk < max_chunks) chunk_region[chunk] = region;
+        chunk[t] = __shfl(id, 0, 64);
+        fill[t] = 0;
+        if (lane == 0 && chunk[t] < max_chunks) chunk_region[chunk[t]] = (qtile0 + t) * ns + static_cast<int>(blockIdx.y);
     };
 
-    auto visit = [&](const floatx16& acc, int v, float wm, const float4* wrow) {
+    auto visit = [&](const floatx16& acc, int v, float wm, const float4* wrow, int t) {
         float lo = acc[0];
 #pragma unroll
         for (int r = 1; r < 16; ++r) lo = __builtin_fminf(lo, acc[r]);  // NaN rows drop out
         const bool other_tile = v >= v_col;  // the same in the whole wave
-        const bool rowp = lo <= Tn;
-        const bool colp = other_tile && lo - wm <= Lq;
+        const bool rowp = lo <= Tn[t];
+        const bool colp = other_tile && lo - wm <= Lq[t];
         if (!__any(rowp || colp)) return;
         const int pt = v >= wrap ? v - wrap : v;
         const uint32_t c0 = static_cast<uint32_t>(pt * 32 + 4 * h);
         if (__any(rowp)) {
             uint32_t mask = 0;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) mask |= (acc[r] <= Tn ? 1u : 0u) << r;
+            for (int r = 0; r < 16; ++r) mask |= (acc[r] <= Tn[t] ? 1u : 0u) << r;
             const int mine = __popc(mask);
             const int other = __shfl_xor(mine, 32, 64);
-            int pos = cnt + (h ? other : 0);
+            int pos = cnt[t] + (h ? other : 0);
             while (mask) {
                 const int r = __ffs(mask) - 1;
                 mask &= mask - 1;
-                if (pos < cap) qbuf[pos] = c0 + static_cast<uint32_t>(8 * (r / 4) + (r % 4));
+                if (pos < cap) qbuf[t][pos] = c0 + static_cast<uint32_t>(8 * (r / 4) + (r % 4));
                 ++pos;
             }
-            cnt += mine + other;
+            cnt[t] += mine + other;
         }
         if (other_tile && __any(colp)) {
 #pragma unroll
@@ -870,32 +953,35 @@ mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2
                 const float wv[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const uint64_t bal = __ballot(acc[4 * b + i] - wv[i] <= Lq);
+                    const uint64_t bal = __ballot(acc[4 * b + i] - wv[i] <= Lq[t]);
                     if (bal == 0) continue;
-                    room_for_two();
+                    room_for_two(t);
                     // lane 0 writes half 0's word (row 8b + i), lane 32 half 1's (row 8b + 4 + i)
-                    const uint32_t lo = static_cast<uint32_t>(bal);
-                    const uint32_t hi = static_cast<uint32_t>(bal >> 32);
-                    const uint32_t m = h ? hi : lo;
+                    const uint32_t lo32 = static_cast<uint32_t>(bal);
+                    const uint32_t hi32 = static_cast<uint32_t>(bal >> 32);
+                    const uint32_t m = h ? hi32 : lo32;
                     if (j == 0 && m != 0) {
-                        const int pos = fill + (h && lo != 0 ? 1 : 0);
+                        const int pos = fill[t] + (h && lo32 != 0 ? 1 : 0);
                         const uint32_t crow = c0 + static_cast<uint32_t>(8 * b + i);
-                        if (chunk < max_chunks) {
-                            pool[static_cast<size_t>(chunk) * kChunkWords + pos] =
+                        if (chunk[t] < max_chunks) {
+                            pool[static_cast<size_t>(chunk[t]) * kChunkWords + pos] =
                                 (static_cast<uint64_t>(crow) << 32) | m;
                         } else {
                             atomicOr(col_over + crow, 1);
                         }
                     }
-                    fill += (lo != 0) + (hi != 0);
+                    fill[t] += (lo32 != 0) + (hi32 != 0);
                 }
             }
         }
     };
-    stream_range<Op, DP, true>(p, bq, j, h, smem, kTilesPerBlock * (B + o0), kTilesPerBlock * (B + o1),
-                               wrap, wcol, wmax, visit);
-    if (valid && h == 0) p.ccount[slot] = cnt;
-    if (lane == 0 && chunk >= 0 && chunk < max_chunks) chunk_count[chunk] = fill;
+    stream_range<Op, DP, true, QT>(p, bq, j, h, smem, kBlockTiles * (B + o0), kBlockTiles * (B + o1), wrap,
+                                   wcol, wmax, visit);
+#pragma unroll
+    for (int t = 0; t < QT; ++t) {
+        if (valid[t] && h == 0) p.ccount[static_cast<size_t>(q[t]) * nslot + blockIdx.y] = cnt[t];
+        if (lane == 0 && chunk[t] >= 0 && chunk[t] < max_chunks) chunk_count[chunk[t]] = fill[t];
+    }
 }
 
 // Column words of each pool chunk (written by query tile t, slice y) into
@@ -939,13 +1025,14 @@ mark_overflow_kernel(const int* __restrict__ col_over, int n, int ns, int cap, i
 
 // Rows whose candidate set would be huge (outliers in sparse space, whose
 // seed threshold covers much of the data) skip the collect passes: the
-// number of m sample rows (every (n/m)-th position, so spread over all cells)
+// number of m sample rows (every (n/m)-th position, so spread over all cells;
+// m = 8192 / d within [64, 512])
 // within the row's seed T estimates its candidate count; above kHeavyCount
 // the row is queued for the seeded list pass while the queue has room, and
 // flagged. In the symmetric pass a flagged row collects nothing and is never
 // a column row, so such rows cannot flood the column buffers. The estimate
 // only steers work; results do not depend on it.
-constexpr int kHeavyCount = 16384;
+constexpr int kHeavyCount = 65536;
 constexpr int kHeavyChunk = 64;  // sample rows staged in LDS at a time
 
 template <int DP>
@@ -1563,7 +1650,8 @@ struct SymBuffers {
 // the selects.
 template <class Op, int DP>
 void launch_sym(const MfmaArgs& a, const SymBuffers& sb, hipStream_t stream) {
-    const int nb = (a.n + 32 * kTilesPerBlock - 1) / (32 * kTilesPerBlock);
+    constexpr int rows_per_block = 32 * kTilesPerBlock * sym_query_tiles<Op, DP>();
+    const int nb = (a.n + rows_per_block - 1) / rows_per_block;
     {
         ProfileScope prof("mfma_collect", stream);
         mfma_sym_collect_kernel<Op, DP><<<dim3(nb, sb.slices), 64 * kTilesPerBlock, 0, stream>>>(
@@ -1679,14 +1767,14 @@ int collect_cap(int) { return kSelectMax / 2; }
 // sets of many thousands, e.g. outliers in high dimension, whose distances
 // concentrate) go to the seeded list sweep over as many slices (kListRows of
 // them), which keeps no candidate buffer at all.
-// Pool chunks of the symmetric pass's column words: 256 words (one chunk)
-// per row, at least 4096 chunks.
-int sym_chunks(int n_points) { return std::max(n_points, 4096); }
+// Pool chunks of the symmetric pass's column words: 256 words (two chunks)
+// per row, at least 8192 chunks.
+int sym_chunks(int n_points) { return std::max(2 * n_points, 8192); }
 
 // Operand rows are padded to whole blocks of query tiles (the symmetric pass
 // pairs blocks; the other kernels ignore the extra rows).
 size_t padded_positions(int n_points) {
-    const size_t b = 32 * kTilesPerBlock;
+    const size_t b = 2 * 32 * kTilesPerBlock;  // blocks of up to two query tiles per wave
     return (static_cast<size_t>(n_points) + b - 1) / b * b;
 }
 
@@ -1732,8 +1820,12 @@ bool sym_enabled() {
 // short tail when the last round of waves is partly filled), with at least 8
 // super-tiles per slice and at most 16 slices (the selects take 32 buffer
 // slices per row).
-int sym_slices(int n_points) {
-    const int nb = (n_points + 32 * kTilesPerBlock - 1) / (32 * kTilesPerBlock);
+// Query tiles per wave of the symmetric pass (sym_query_tiles on the host).
+int sym_qt(int dp, int form) { return form == kFormBF16 && dp >= 64 ? 2 : 1; }
+
+int sym_slices(int n_points, int qt) {
+    const int rows_per_block = 32 * kTilesPerBlock * qt;
+    const int nb = (n_points + rows_per_block - 1) / rows_per_block;
     const int by_waves = (16384 + kTilesPerBlock * nb - 1) / (kTilesPerBlock * nb);
     const int by_len = std::max(1, (nb / 2 + 1) / 8);
     return std::max(1, std::min(std::min(by_waves, by_len), 16));
@@ -1749,7 +1841,7 @@ size_t mfma_scratch_bytes(int n_points, int dp, int form, int rows, int k, bool 
     if (collect) {
         const size_t slots = static_cast<size_t>(rows) * s;
         const size_t second = static_cast<size_t>(std::min(rows, kSecondRows)) * second_slices(n_points);
-        const size_t ss = static_cast<size_t>(sym_slices(n_points));
+        const size_t ss = static_cast<size_t>(sym_slices(n_points, sym_qt(dp, form)));
         const size_t sym = static_cast<size_t>(rows) * (2 * ss * (kSymCap + 1) + 2) * sizeof(uint32_t) +
                            padded * sizeof(float) + padded / 32 * sizeof(float2) +
                            static_cast<size_t>(sym_chunks(n_points)) * (kChunkWords * sizeof(uint64_t) + 2 * sizeof(int)) +
@@ -1802,7 +1894,7 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
     int* heavy = nullptr;
     if (collect) {
         if (sym) {
-            sb.slices = sym_slices(n_points);
+            sb.slices = sym_slices(n_points, sym_qt(dp, form));
             args.slices = 2 * sb.slices;
             args.cap = kSymCap;
             const size_t slots = static_cast<size_t>(rows) * args.slices;
@@ -1884,7 +1976,9 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
         {
             // Rows with huge candidate sets go to the list pass up front.
             ProfileScope prof("heavy_rows", stream);
-            const int m = std::min(n_points, dp <= 32 ? 512 : 256);
+            // ~8k sample features per row (64..512 rows): the estimate only
+            // has to tell rows near kHeavyCount candidates from rows near n.
+            const int m = std::min(n_points, std::clamp(8192 / dp, 64, 512));
             auto go = [&](auto D) {
                 heavy_rows_kernel<decltype(D)::value><<<(n_points + 255) / 256, 256, 0, stream>>>(
                     ord->xs_ord, ord->order, tseed, n_points, m, heavy, second.ov_rows, second.ov_count,
