@@ -1287,13 +1287,22 @@ collect_select_kernel(MfmaArgs p) {
         if (lane == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
         return;
     }
-    // Exact keys straight from the slice buffers, slice by slice.
-    for (int b = 0; b < nb; ++b) {
-        const int b0 = s_pre[wave][b];
-        const int cb = s_pre[wave][b + 1] - b0;
-        const uint32_t* src = p.cbuf + (base + b) * p.cap;
+    // Exact keys straight from the slice buffers, every lane busy: entry e's
+    // slice is the last b with s_pre[b] <= e (a binary search; empty slices
+    // share their prefix with the next one).
 #pragma unroll 1
-        for (int i = lane; i < cb; i += 64) s_words[wave][b0 + i] = collected_word<DP>(p, s_q[wave], src[i], qi);
+    for (int e = lane; e < total; e += 64) {
+        int lo = 0, hi = nb;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_pre[wave][mid] <= e) {
+                lo = mid;
+            } else {
+                hi = mid;
+            }
+        }
+        const uint32_t pos = p.cbuf[(base + lo) * p.cap + (e - s_pre[wave][lo])];
+        s_words[wave][e] = collected_word<DP>(p, s_q[wave], pos, qi);
     }
     wave_lds_sync();
     if (total <= 128) {
