@@ -1821,6 +1821,7 @@ int second_slices(int n_points) {
 // slice goes to the second pass.
 // CULOF_SYM=0 turns it off (the same results; for measurement).
 constexpr int kSymCap = 1024;
+constexpr int kSymMinRows = 32768;
 bool sym_enabled() {
     const char* env = std::getenv("CULOF_SYM");
     return !(env && env[0] == '0');
@@ -1898,7 +1899,10 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
     MfmaArgs second = args;
     MfmaArgs list = args;
     // A full run in candidate order computes each pair of tiles once.
-    const bool sym = collect && ord && qlist == ord->order && row0 == 0 && rows == n_points && sym_enabled();
+    // (Below kSymMinRows the extra launches of the symmetric pass cost more
+    // than the halved matrix work saves.)
+    const bool sym = collect && ord && qlist == ord->order && row0 == 0 && rows == n_points &&
+                     n_points >= kSymMinRows && sym_enabled();
     SymBuffers sb{};
     int* heavy = nullptr;
     if (collect) {

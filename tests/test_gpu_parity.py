@@ -239,14 +239,17 @@ def test_order_switch_gives_same_bits(monkeypatch):
         np.testing.assert_array_equal(default[key], plain[key], err_msg=key)
 
 
-@pytest.mark.parametrize("n,d,k,form", [(20_000, 8, 20, 0), (16_384, 16, 10, 1), (5_000, 64, 31, 0),
-                                         (9_100, 32, 15, 1), (300, 4, 7, 0)])
+@pytest.mark.parametrize("n,d,k,form", [(40_000, 8, 20, 0), (36_000, 16, 10, 1), (33_000, 64, 31, 0),
+                                         (35_840, 64, 16, 1), (34_000, 128, 20, 1), (35_000, 32, 15, 1),
+                                         (50_000, 4, 7, 0)])
 def test_symmetric_pass_switch_gives_same_bits(monkeypatch, n, d, k, form):
     """The symmetric first pass (each pair of tiles computed once, the screen
-    value tested against both rows' thresholds) and CULOF_SYM=0 (every query
-    row against every candidate tile) agree bit for bit, on clustered data
-    with far outliers (large norms next to small thresholds), for odd and even
-    block counts; sampled rows also match the oracle."""
+    value tested against both rows' thresholds; full runs of at least 32,768
+    rows) and CULOF_SYM=0 (every query row against every candidate tile) agree
+    bit for bit, on clustered data with far outliers (large norms next to small
+    thresholds), for odd and even block counts and both query-tile widths
+    (two per wave for the wide BF16 screen); sampled rows also match the
+    oracle."""
     from _gpu import knn
 
     rng = np.random.default_rng(n + d)
