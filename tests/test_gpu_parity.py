@@ -267,6 +267,33 @@ def test_symmetric_pass_switch_gives_same_bits(monkeypatch, n, d, k, form):
         _assert_same_bits(sym[2][r], ok[0], f"row {r} kdist2")
 
 
+@pytest.mark.parametrize("form", [0, 1], ids=["fp32", "bf16"])
+def test_overflow_guard_in_symmetric_pass(form):
+    """Norms beyond FLT_MAX / 16 set the device guard: the symmetric pass and
+    its scatter and selects stand down and every row takes the radix path
+    (keys stay finite at this scale); sampled rows match the oracle."""
+    from _gpu import knn
+
+    rng = np.random.default_rng(91)
+    n, d, k = 36_000, 4, 10
+    X = (rng.standard_normal((n, d)) * 9.5e17).astype(np.float32)  # one norm past the guard; keys < FLT_MAX
+    from culof_amd import _native
+
+    _native.profile_reset()
+    _native.profile_enable(True)
+    try:
+        gi, gd, gk = knn(X, k, form=form)
+    finally:
+        _native.profile_enable(False)
+    # every row went through the radix redo, which costs far more than the
+    # stood-down collect pass
+    assert _native.profile_read("radix_knn")[0] > 5 * _native.profile_read("mfma_collect")[0]
+    for r in rng.choice(n, 6, replace=False):
+        oi, od, ok = oracle.knn_rows(X, k, int(r), 1)
+        np.testing.assert_array_equal(gi[:, r], oi[0], err_msg=f"row {r}")
+        _assert_same_bits(gk[r], ok[0], f"row {r} kdist2")
+
+
 def test_every_fallback_tier():
     """10,000 copies of one point among 150,000: their collected sets overflow
     the first pass, the second pass and the list queue, so the rows run
