@@ -1,30 +1,36 @@
-// MFMA-screened exact kNN sweep: the production selection for every d <= 128
-// and k <= 64, in two precision forms of the screen.
+// MFMA-screened exact kNN: the production selection for every d <= 128 and
+// k <= 64, in two precision forms of the screen.
 //
 // Same result as sweep.hip and radix.hip -- the k smallest difference-form
 // FP32 keys e(i, j) = sum_t (a_t - b_t)^2 per query, ties toward the lower
-// index, slot order of the reference (src/cuda/knn.cu:190-224) -- computed with
-// the distance screen on the matrix cores:
+// index, slot order of the reference (src/cuda/knn.cu:190-224) -- with the
+// distance screen on the matrix cores:
 //
-//   * A wave owns 32 queries; lanes l and l + 32 both serve query l % 32.
-//     The 4 waves of a block hold 4 query tiles and stream the same candidate
-//     tiles, so a tile fetched by one wave is an L1 hit for the others.
 //   * Per 32 x 32 tile the screen is one chain of MFMAs, C = m + A B with
 //     A = candidate rows (32 x DP), B = -2 x query rows (DP x 32) and the
 //     accumulator pre-loaded with m_i = |x_i|^2 (1 - c). Form FP32 uses
 //     v_mfma_f32_32x32x2_f32 on the centred rows; form BF16 uses
-//     v_mfma_f32_32x32x16_bf16 on RNE-rounded copies (products exact,
-//     FP32 accumulation). The query's own |x_q|^2 is folded into the threshold,
-//     so the VALU does one min per pair and one compare per tile.
-//   * A pair passes the screen when acc <= T (1 + c) + c |x_q|^2 - |x_q|^2, with
-//     T the lane's current k-th exact key and c the form's relative error bound
-//     (derivation below). Passing candidates are queued (index only) in a
-//     per-lane LDS ring; when some lane's ring is nearly full the wave computes
-//     their exact keys in a batch and inserts them into the lane's sorted
-//     register list, exactly as the VALU sweep does.
-//   * Grid = (query tiles / 4) x S candidate slices. With S > 1 each slice
-//     writes its per-query top-k as (key, index) words and merge_slices_kernel
-//     folds them; with S = 1 the sweep writes the final lists directly.
+//     v_mfma_f32_32x32x16_bf16 on split hi / lo copies (products exact,
+//     FP32 accumulation). The query's own |x_q|^2 is folded into the
+//     threshold, so the VALU does one min per pair and one compare per tile.
+//     A wave owns 32 queries (lanes l and l + 32 serve query l % 32); the 4
+//     waves of a block stream the same candidate tiles through LDS.
+//   * A pair passes when acc <= T (1 + c) + c |x_q|^2 - |x_q|^2, c the form's
+//     relative error bound (below), so a candidate with exact key <= T always
+//     passes. Three ways use it:
+//     - mfma_sym_collect_kernel (full runs of 32,768 rows or more): T is a
+//       seed (an upper bound of the k-th key), each unordered pair of tiles is
+//       computed once and tested against both rows' thresholds, and passing
+//       candidates are appended to per-row buffers;
+//     - mfma_collect_kernel (other runs, the second pass, near-tie rows): the
+//       same against every candidate tile of the query's slice;
+//     - mfma_sweep_kernel (CULOF_ORDER=0, and the list pass for rows whose
+//       candidate sets outgrow the buffers): T is the lane's current k-th key,
+//       passing candidates are queued in a per-lane LDS ring and inserted into
+//       a sorted register list by exact key, slices merged by
+//       merge_slices_kernel.
+//     The collect kernels leave the exact keys to the select kernels
+//     (collect_select_kernel, big_select_kernel).
 //
 // Screen bound. Let u = 2^-24, N the exact squared norms of the centred rows.
 // The accumulated value differs from m_i - 2 x_q.x_i by at most 2 g (N_q + N_i),
