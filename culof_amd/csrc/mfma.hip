@@ -1131,6 +1131,13 @@ column_maxima_kernel(const float* __restrict__ wcol, int tiles, float2* __restri
 // Collected entries per query the wave select holds; larger sets go to the
 // block select, which streams any number through LDS.
 constexpr int kSelectMax = 1024;
+// The wave select's own capacity: narrow rows (short candidate lists) take
+// 512, which halves its LDS and doubles the waves per CU; their rare longer
+// lists go to the block select.
+template <int DP>
+constexpr int wave_select_max() {
+    return DP <= 16 ? 512 : kSelectMax;
+}
 constexpr int kBigChunk = 2048;  // words sorted per step of the block select
 
 
@@ -1237,7 +1244,7 @@ __device__ __forceinline__ int query_of(const MfmaArgs& p, int local) {
     return p.qlist ? p.qlist[local] : p.row0 + local;
 }
 
-// One wave per query of the collect pass: sets of up to kSelectMax entries
+// One wave per query of the collect pass: sets of up to wave_select_max<DP>() entries
 // are keyed and selected here; larger ones are queued for the block select
 // (big_rows) or, when a slice buffer overflowed, for a second collect pass
 // with larger buffers (ov_rows).
@@ -1245,7 +1252,8 @@ template <int DP>
 __global__ void __launch_bounds__(256)
 collect_select_kernel(MfmaArgs p) {
     __shared__ __attribute__((aligned(16))) float s_q[4][DP];
-    __shared__ uint64_t s_words[4][kSelectMax];
+    constexpr int kWaveMax = wave_select_max<DP>();
+    __shared__ uint64_t s_words[4][kWaveMax];
     __shared__ uint64_t s_sel[4][64];
     __shared__ int s_pre[4][33];
     const int lane = threadIdx.x & 63;
@@ -1285,7 +1293,7 @@ collect_select_kernel(MfmaArgs p) {
         }
         return;
     }
-    if (total > kSelectMax) {
+    if (total > kWaveMax) {
         if (lane == 0) p.big_rows[atomicAdd(p.big_count, 1)] = local;
         return;
     }
@@ -1315,9 +1323,9 @@ collect_select_kernel(MfmaArgs p) {
         select_words<2>(p, s_words[wave], total, qi, lane, s_sel[wave]);
     } else if (total <= 256) {
         select_words<4>(p, s_words[wave], total, qi, lane, s_sel[wave]);
-    } else if (total <= 512) {
+    } else if (kWaveMax <= 512 || total <= 512) {
         select_words<8>(p, s_words[wave], total, qi, lane, s_sel[wave]);
-    } else {
+    } else if constexpr (kWaveMax > 512) {
         select_words<16>(p, s_words[wave], total, qi, lane, s_sel[wave]);
     }
 }
