@@ -1136,7 +1136,7 @@ constexpr int kSelectMax = 1024;
 // lists go to the block select.
 template <int DP>
 constexpr int wave_select_max() {
-    return DP <= 16 ? 512 : kSelectMax;
+    return DP <= 32 ? 512 : kSelectMax;
 }
 constexpr int kBigChunk = 2048;  // words sorted per step of the block select
 
