@@ -1150,29 +1150,49 @@ __device__ __forceinline__ uint64_t collected_word(const MfmaArgs& p, const floa
     return pack(kb, orig);
 }
 
-// The k smallest words, given in ascending order in `sorted` (k of them), or
-// any order in `sel` with the k-th word `kth_word`: written in the reference's
-// slot order by lanes 0..k-1 of the calling wave.
+// The k smallest words of a row, in any order in sel[0, k) (k <= 64): written
+// in the reference's slot order by the calling wave (all 64 lanes).
 __device__ void emit_selected(const MfmaArgs& p, const uint64_t* sel, int qi, int lane) {
+    // Called by a whole wave. Lane f holds word f; the k-th word is their
+    // maximum. Sorting the lanes by (group, index) -- group 0 for keys below
+    // the k-th, 1 for keys equal to it, 2 for lanes past k -- puts every word
+    // at its slot: below-k-th keys in ascending index, then the tied ones.
     const int k = p.k;
-    uint64_t kw = 0;
-    for (int f = 0; f < k; ++f) kw = sel[f] > kw ? sel[f] : kw;
+    const bool valid = lane < k;
+    const uint64_t me = valid ? sel[lane] : 0;
+    uint64_t kw = me;
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        const uint32_t hi = static_cast<uint32_t>(__shfl_xor(static_cast<int>(kw >> 32), m, 64));
+        const uint32_t lo = static_cast<uint32_t>(__shfl_xor(static_cast<int>(kw), m, 64));
+        const uint64_t o = (static_cast<uint64_t>(hi) << 32) | lo;
+        kw = o > kw ? o : kw;
+    }
     const uint32_t kth = static_cast<uint32_t>(kw >> 32);
-    int below = 0;
-    for (int f = 0; f < k; ++f) below += static_cast<uint32_t>(sel[f] >> 32) < kth;
-    if (lane < k) {
-        const uint64_t me = sel[lane];
-        const uint32_t key = static_cast<uint32_t>(me >> 32);
-        const uint32_t idx = static_cast<uint32_t>(me);
-        int slot = key < kth ? 0 : below;
-        for (int f = 0; f < k; ++f) {
-            const uint64_t o = sel[f];
-            const uint32_t ok = static_cast<uint32_t>(o >> 32);
-            slot += ((key < kth) ? (ok < kth) : (ok == kth)) && static_cast<uint32_t>(o) < idx;
+    uint32_t key = static_cast<uint32_t>(me >> 32);
+    const uint32_t grp = !valid ? 2u : (key < kth ? 0u : 1u);
+    uint64_t s = (static_cast<uint64_t>(grp) << 32) | static_cast<uint32_t>(me);
+#pragma unroll
+    for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const uint32_t ohi = static_cast<uint32_t>(__shfl_xor(static_cast<int>(s >> 32), stride, 64));
+            const uint32_t olo = static_cast<uint32_t>(__shfl_xor(static_cast<int>(s), stride, 64));
+            const uint32_t okey = static_cast<uint32_t>(__shfl_xor(static_cast<int>(key), stride, 64));
+            const uint64_t os = (static_cast<uint64_t>(ohi) << 32) | olo;
+            const bool up = (lane & size) == 0;
+            const bool lower = (lane & stride) == 0;
+            const bool swap = (lower == up) ? (os < s) : (os > s);
+            if (swap) {
+                s = os;
+                key = okey;
+            }
         }
+    }
+    if (valid) {
         const size_t ns = static_cast<size_t>(p.n);
-        p.nbr_idx[static_cast<size_t>(slot) * ns + qi] = static_cast<int>(idx);
-        p.nbr_d2[static_cast<size_t>(slot) * ns + qi] = __uint_as_float(key);
+        p.nbr_idx[static_cast<size_t>(lane) * ns + qi] = static_cast<int>(static_cast<uint32_t>(s));
+        p.nbr_d2[static_cast<size_t>(lane) * ns + qi] = __uint_as_float(key);
         if (lane == 0) p.kdist2[qi] = __uint_as_float(kth);
     }
 }
