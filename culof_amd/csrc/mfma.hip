@@ -1209,8 +1209,9 @@ __device__ void select_words(const MfmaArgs& p, const uint64_t* s_words, int tot
     }
     // The k-th smallest word: the k-th key by a bitwise search on the key
     // bits, then, only when that key is tied, the right index among the tied.
+    // (Keys are at most +inf, bit 31 clear; at least k of them are real.)
     uint32_t kh = 0;
-    for (int b = 31; b >= 0; --b) {
+    for (int b = 30; b >= 0; --b) {
         const uint32_t probe = kh | (1u << b);
         int below = 0;
 #pragma unroll
