@@ -1351,29 +1351,38 @@ collect_select_kernel(MfmaArgs p) {
     }
 }
 
-// One block per queued row (grid-stride): the row's collected entries in
-// steps of kBigChunk - k, each step sorting the new words together with the
-// best k so far (bitonic, in LDS) and keeping k. Any number of entries.
+// One block per queued row (grid-stride). The row's collected entries are
+// keyed in rounds that fill the free part of an LDS buffer of kBigChunk words;
+// once the buffer has been sorted, only words below the running k-th word
+// (the k-th smallest seen so far; the words are distinct) enter it, so later
+// rounds mostly key and drop. The buffer is sorted (bitonic, over the next
+// power of two it fills) when less than kBigMinRoom words are free or the
+// entries run out, keeping the k smallest. Any number of entries.
 // rows_list[r] is the row's local index in the collect pass it came from
 // (null: r itself); that pass's buffers are p.cbuf / p.ccount / p.cap.
+constexpr int kBigThreads = 512;
+constexpr int kBigMinRoom = 512;
+
 template <int DP>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kBigThreads)
 big_select_kernel(MfmaArgs p, const int* __restrict__ rows_list, const int* __restrict__ rows_count) {
     __shared__ __attribute__((aligned(16))) float s_q[DP];
     __shared__ uint64_t s_w[kBigChunk];
     __shared__ int s_pre[33];
+    __shared__ int s_fill;
     if (*p.guard) return;
     const int count = min(*rows_count, p.rows);
     const int nb = p.slices;
+    const int tid = threadIdx.x;
     for (int r = blockIdx.x; r < count; r += gridDim.x) {
         const int local = rows_list ? rows_list[r] : r;
         const int qi = query_of(p, local);
         const size_t base = static_cast<size_t>(local) * nb;
         __syncthreads();
-        if (threadIdx.x < nb) s_pre[threadIdx.x + 1] = p.ccount[base + threadIdx.x];
-        for (int t = threadIdx.x; t < DP; t += 256) s_q[t] = p.xs[static_cast<size_t>(qi) * DP + t];
+        if (tid < nb) s_pre[tid + 1] = p.ccount[base + tid];
+        for (int t = tid; t < DP; t += kBigThreads) s_q[t] = p.xs[static_cast<size_t>(qi) * DP + t];
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (tid == 0) {
             int run = 0;
             s_pre[0] = 0;
             bool over = false;
@@ -1388,7 +1397,7 @@ big_select_kernel(MfmaArgs p, const int* __restrict__ rows_list, const int* __re
         if (s_pre[0] < 0 || s_pre[nb] < p.k) {
             // A slice buffer overflowed: the seeded list sweep redoes the row
             // when a queue is given (radix when it is full or absent).
-            if (threadIdx.x == 0) {
+            if (tid == 0) {
                 const int at = (s_pre[0] < 0 && p.ov_rows) ? atomicAdd(p.ov_count, 1) : p.ov_cap;
                 if (at < p.ov_cap) {
                     p.ov_rows[at] = qi;
@@ -1399,24 +1408,46 @@ big_select_kernel(MfmaArgs p, const int* __restrict__ rows_list, const int* __re
             continue;
         }
         const int total = s_pre[nb];
-        int kept = 0;  // best words so far at s_w[0..kept)
-        for (int e0 = 0; e0 < total;) {
-            const int take = min(kBigChunk - kept, total - e0);
-            for (int i = threadIdx.x; i < kBigChunk - kept; i += 256) {
-                uint64_t w = kSentinel;
-                if (i < take) {
+        int used = 0;               // words at s_w[0, used)
+        bool sorted = false;        // s_w[0, used) is the k smallest so far, used == k
+        uint64_t thr = kSentinel;   // after a sort: the k-th smallest word so far
+        for (int e0 = 0; e0 < total || !sorted;) {
+            if (e0 < total) {
+                const int take = min(kBigChunk - used, total - e0);
+                if (tid == 0) s_fill = used;
+                __syncthreads();
+                for (int i = tid; i < take; i += kBigThreads) {
                     const int e = e0 + i;
-                    int b = 0;
-                    while (e >= s_pre[b + 1]) ++b;
-                    const uint32_t pos = p.cbuf[(base + b) * p.cap + (e - s_pre[b])];
-                    w = collected_word<DP>(p, s_q, pos, qi);
+                    // entry e's slice: the last b with s_pre[b] <= e
+                    int lo = 0, hi = nb;
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (s_pre[mid] <= e) {
+                            lo = mid;
+                        } else {
+                            hi = mid;
+                        }
+                    }
+                    const uint32_t pos = p.cbuf[(base + lo) * p.cap + (e - s_pre[lo])];
+                    const uint64_t w = collected_word<DP>(p, s_q, pos, qi);
+                    if (w < thr) s_w[atomicAdd(&s_fill, 1)] = w;
                 }
-                s_w[kept + i] = w;
+                __syncthreads();
+                const int grown = s_fill;
+                sorted = sorted && grown == used;
+                used = grown;
+                e0 += take;
+                if (e0 < total && kBigChunk - used >= kBigMinRoom) continue;
+                if (sorted) continue;
             }
+            // Sort s_w[0, used) (padded to a power of two) and keep the k smallest.
+            int m = 64;
+            while (m < used) m <<= 1;
+            for (int i = used + tid; i < m; i += kBigThreads) s_w[i] = kSentinel;
             __syncthreads();
-            for (int size = 2; size <= kBigChunk; size <<= 1) {
+            for (int size = 2; size <= m; size <<= 1) {
                 for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                    for (int q = threadIdx.x; q < kBigChunk / 2; q += 256) {
+                    for (int q = tid; q < m / 2; q += kBigThreads) {
                         const int lo = 2 * q - (q & (stride - 1));
                         const int hi = lo + stride;
                         const bool up = (lo & size) == 0;
@@ -1429,14 +1460,15 @@ big_select_kernel(MfmaArgs p, const int* __restrict__ rows_list, const int* __re
                     __syncthreads();
                 }
             }
-            kept = p.k;
-            e0 += take;
+            used = p.k;
+            thr = s_w[p.k - 1];
+            sorted = true;
         }
-        if (threadIdx.x < 64) {
+        if (tid < 64) {
             if (static_cast<uint32_t>(s_w[p.k - 1] >> 32) >= kInfBits) {
-                if (threadIdx.x == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+                if (tid == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
             } else {
-                emit_selected(p, s_w, qi, threadIdx.x);
+                emit_selected(p, s_w, qi, tid);
             }
         }
     }
@@ -1670,10 +1702,10 @@ void launch_collect(const MfmaArgs& a, hipStream_t stream) {
     if (a.big_rows) {
         // First pass: a wave per row, large sets to the block select.
         collect_select_kernel<DP><<<(a.rows + 3) / 4, 256, 0, stream>>>(a);
-        big_select_kernel<DP><<<big_grid, 256, 0, stream>>>(a, a.big_rows, a.big_count);
+        big_select_kernel<DP><<<big_grid, kBigThreads, 0, stream>>>(a, a.big_rows, a.big_count);
     } else {
         // Second pass over overflowed rows: all large.
-        big_select_kernel<DP><<<big_grid, 256, 0, stream>>>(a, nullptr, a.row_count);
+        big_select_kernel<DP><<<big_grid, kBigThreads, 0, stream>>>(a, nullptr, a.row_count);
     }
 }
 
@@ -1711,7 +1743,7 @@ void launch_sym(const MfmaArgs& a, const SymBuffers& sb, hipStream_t stream) {
     }
     ProfileScope prof("collect_select", stream);
     collect_select_kernel<DP><<<(a.rows + 3) / 4, 256, 0, stream>>>(a);
-    big_select_kernel<DP><<<std::min(a.rows, 1024), 256, 0, stream>>>(a, a.big_rows, a.big_count);
+    big_select_kernel<DP><<<std::min(a.rows, 1024), kBigThreads, 0, stream>>>(a, a.big_rows, a.big_count);
 }
 
 template <class Op, int DP>
