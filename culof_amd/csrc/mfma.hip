@@ -262,6 +262,9 @@ struct ChunkShape {
     // the symmetric pass also stages the rows' column thresholds (4 bytes per
     // row) and their per-(tile, half) maxima (16 bytes per tile, 8 used)
     static constexpr size_t kSymBytes = static_cast<size_t>(kTiles) * (kTileBytes + 32 * 4 * 2 + 16);
+    // Streamed by LDS DMA (two query tiles per wave): two tiles per chunk.
+    static constexpr int kDmaTiles = 2;
+    static constexpr size_t kDmaSymBytes = static_cast<size_t>(kDmaTiles) * (kTileBytes + 32 * 4 * 2 + 16);
 };
 
 // Exact key of a query row against key row xs_row: the same FMA chain as the
@@ -393,7 +396,10 @@ __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename O
     using Frag = typename Op::template Frag<DP>;
     using Elem = typename Op::Elem;
     using Shape = ChunkShape<Op, DP>;
-    constexpr int CT = Shape::kTiles;
+    // Two query tiles per wave: the A tiles go global -> LDS by DMA, two per
+    // chunk (no staging registers).
+    constexpr bool kDma = QT == 2;
+    constexpr int CT = kDma ? Shape::kDmaTiles : Shape::kTiles;
     constexpr int kRowsPerChunk = 32 * CT;
     constexpr int kRowF4 = Shape::kRowF4;     // float4s per global row (DP * elem >= 16 bytes)
     constexpr int kPitchF4 = Shape::kPitchF4; // float4s per LDS row
@@ -401,7 +407,7 @@ __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename O
     constexpr int kChunkF4 = CT * kTileF4;
     constexpr int kLdsTileF4 = 32 * kPitchF4;
     constexpr int kLdsChunkF4 = CT * kLdsTileF4;
-    constexpr int kLoadsA = (kChunkF4 + 255) / 256;  // per thread
+    constexpr int kLoadsA = ((kDma ? kLdsChunkF4 : kChunkF4) + 255) / 256;  // per thread
     constexpr int kChunkM4 = kRowsPerChunk / 4;       // float4s of norms per chunk
     static_assert(DP * Op::kRowMul * sizeof(Elem) % 16 == 0, "screen rows must be whole float4s");
     static_assert(2 * kChunkM4 + CT <= 256, "one staging thread per float4");
@@ -417,21 +423,62 @@ __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename O
     const float nan = __uint_as_float(0x7FC00000u);
     auto phys = [&](int v) { return v >= wrap ? v - wrap : v; };
 
-    float4 ra[kLoadsA];
+    float4 ra[kDma ? 1 : kLoadsA];
+    // Separate registers for norms and column thresholds: one register
+    // loaded by both would make the wave wait for the whole fetch right after
+    // issuing it (a load may not overwrite a register another load still owns).
     float4 rm = make_float4(nan, nan, nan, nan);
+    float4 rw = make_float4(nan, nan, nan, nan);
     float2 rx = make_float2(nan, nan);
     // threads [0, kChunkM4) stage norms, [kChunkM4, 2 kChunkM4) column
     // thresholds, [2 kChunkM4, 2 kChunkM4 + CT) the per-tile maxima
     const int tid = threadIdx.x;
-    auto fetch = [&](int t0) {
+    // dbuf: the LDS buffer the DMA writes (kDma only)
+    auto fetch = [&](int t0, int dbuf) {
+        if constexpr (kDma) {
+            // Instruction u of wave w fills LDS float4s [256 u + 64 w, +64),
+            // one per lane, in the padded layout: slot idx is float4
+            // idx % kPitchF4 of chunk row idx / kPitchF4. Lanes on a pad slot,
+            // past the chunk or on a tile past t_end write nothing: those
+            // slots are never read, or only by tiles that are never visited.
+            float4* dst = s_a + dbuf * kLdsChunkF4 + (tid & ~63);
 #pragma unroll
-        for (int u = 0; u < kLoadsA; ++u) {
-            const int idx = tid + 256 * u;
-            const int vt = t0 + idx / kTileF4;
-            const bool live = idx < kChunkF4 && vt < t_end;
-            ra[u] = live ? ga[static_cast<size_t>(phys(vt)) * kTileF4 + idx % kTileF4]
-                         : make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int u = 0; u < kLoadsA; ++u) {
+                const int idx = tid + 256 * u;
+                const int row = idx / kPitchF4;
+                const int f = idx - row * kPitchF4;
+                const int vt = t0 + row / 32;
+                if (idx < kLdsChunkF4 && f < kRowF4 && vt < t_end)
+                    __builtin_amdgcn_global_load_lds(
+                        ga + static_cast<size_t>(phys(vt)) * kTileF4 + (row % 32) * kRowF4 + f, dst + 256 * u, 16,
+                        0, 0);
+            }
+            // Wave 0 also moves the chunk's norms, column thresholds (lanes
+            // [0, kChunkM4), one float4 each) and per-tile maxima (lanes
+            // [0, 2 CT), one float each).
+            if (tid < 64) {
+                const int vt = t0 + tid / 8;
+                if (tid < kChunkM4 && vt < t_end) {
+                    const size_t at = static_cast<size_t>(phys(vt)) * 8 + tid % 8;
+                    __builtin_amdgcn_global_load_lds(gm + at, s_m + dbuf * kChunkM4, 16, 0, 0);
+                    __builtin_amdgcn_global_load_lds(gw + at, s_w + dbuf * kChunkM4, 16, 0, 0);
+                }
+                const int vx = t0 + tid / 2;
+                if (tid < 2 * CT && vx < t_end)
+                    __builtin_amdgcn_global_load_lds(reinterpret_cast<const float*>(wmax + phys(vx)) + (tid & 1),
+                                                     s_x + dbuf * CT, 4, 0, 0);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < kLoadsA; ++u) {
+                const int idx = tid + 256 * u;
+                const int vt = t0 + idx / kTileF4;
+                const bool live = idx < kChunkF4 && vt < t_end;
+                ra[u] = live ? ga[static_cast<size_t>(phys(vt)) * kTileF4 + idx % kTileF4]
+                             : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
         }
+        if constexpr (kDma) return;
         if (tid < kChunkM4) {
             const int vt = t0 + tid / 8;
             rm = vt < t_end ? gm[static_cast<size_t>(phys(vt)) * 8 + tid % 8] : make_float4(nan, nan, nan, nan);
@@ -440,7 +487,7 @@ __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename O
             if (tid >= kChunkM4 && tid < 2 * kChunkM4) {
                 const int e = tid - kChunkM4;
                 const int vt = t0 + e / 8;
-                rm = vt < t_end ? gw[static_cast<size_t>(phys(vt)) * 8 + e % 8] : make_float4(nan, nan, nan, nan);
+                rw = vt < t_end ? gw[static_cast<size_t>(phys(vt)) * 8 + e % 8] : make_float4(nan, nan, nan, nan);
             }
             if (tid >= 2 * kChunkM4 && tid < 2 * kChunkM4 + CT) {
                 const int vt = t0 + tid - 2 * kChunkM4;
@@ -449,6 +496,7 @@ __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename O
         }
     };
     auto stage_chunk = [&](int buf) {
+        if constexpr (kDma) return;  // the DMA wrote the buffer
 #pragma unroll
         for (int u = 0; u < kLoadsA; ++u) {
             const int idx = tid + 256 * u;
@@ -457,7 +505,7 @@ __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename O
         }
         if (tid < kChunkM4) s_m[buf * kChunkM4 + tid] = rm;
         if constexpr (kSym) {
-            if (tid >= kChunkM4 && tid < 2 * kChunkM4) s_w[buf * kChunkM4 + tid - kChunkM4] = rm;
+            if (tid >= kChunkM4 && tid < 2 * kChunkM4) s_w[buf * kChunkM4 + tid - kChunkM4] = rw;
             if (tid >= 2 * kChunkM4 && tid < 2 * kChunkM4 + CT) s_x[buf * CT + tid - 2 * kChunkM4] = rx;
         }
     };
@@ -483,15 +531,21 @@ __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename O
         }
     };
 
+    // Every DMA of a chunk has landed once its issuing wave waits for its
+    // memory counter; the barrier after that publishes the buffer.
+    auto dma_wait = [&]() {
+        if constexpr (kDma) __builtin_amdgcn_s_waitcnt(0);
+    };
     if (n_chunks > 0) {
-        fetch(t_begin);
+        fetch(t_begin, 0);
         stage_chunk(0);
     }
+    dma_wait();
     __syncthreads();
     for (int ch = 0; ch < n_chunks; ++ch) {
         const int buf = ch & 1;
         const int t0 = t_begin + ch * CT;
-        if (ch + 1 < n_chunks) fetch(t0 + CT);  // next chunk in flight
+        if (ch + 1 < n_chunks) fetch(t0 + CT, buf ^ 1);  // next chunk in flight
         const float4* sa = s_a + buf * kLdsChunkF4;
         const float4* sm = s_m + buf * kChunkM4;
         if constexpr (kSym) {
@@ -539,6 +593,7 @@ __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename O
             }
         }
         if (ch + 1 < n_chunks) stage_chunk(buf ^ 1);
+        dma_wait();
         __syncthreads();
     }
 }
@@ -856,7 +911,8 @@ mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2
     using Frag = typename Op::template Frag<DP>;
     constexpr int QT = sym_query_tiles<Op, DP>();
     constexpr int kBlockTiles = kTilesPerBlock * QT;
-    __shared__ __attribute__((aligned(16))) unsigned char smem[2 * ChunkShape<Op, DP>::kSymBytes];
+    __shared__ __attribute__((aligned(16)))
+    unsigned char smem[2 * (QT == 2 ? ChunkShape<Op, DP>::kDmaSymBytes : ChunkShape<Op, DP>::kSymBytes)];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
