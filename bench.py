@@ -21,6 +21,9 @@ torch.cuda.synchronize(). Prints one JSON line:
                full workload (the reference's own comparator), with agreement
 * parity       the sampled rows' neighbour lists and k-distances compared bit
                for bit with the oracle, and scores against scikit-learn
+* host_path    culof_amd.lof on the host array, best of 7 (PCIe copies
+               included; the convention the reference's published figure
+               uses), with its scores compared to the device path's
 * fwd_bwd      forward + hand-written backward of sum(w * LOF) at n=50k, d=32,
                k=20 (points/s of both passes, and the ratio to the forward)
 * forms        the BF16-screen form against the plain FP32 form at n=100k,
@@ -339,6 +342,24 @@ def main() -> int:
         parity["vs_sklearn_top1pct_agreement"] = agree
         log(f"[bench] sklearn {sk_s:.2f} s; agreement {parity}")
 
+    # ---- host arrays in and out (the reference's own timing convention:
+    # culof.lof on a numpy array, best of 7 after a warm-up); never `value` ----
+    host = np.ascontiguousarray(X, dtype=np.float32)
+    culof_amd.lof(host, k)
+    host_best = float("inf")
+    for _ in range(7):
+        t = time.perf_counter()
+        host_scores = culof_amd.lof(host, k)
+        host_best = min(host_best, time.perf_counter() - t)
+    host_path = {"value": round(n / host_best, 2), "unit": "points/s", "ms": round(host_best * 1e3, 3),
+                 "scores_bit_identical_to_device_path": bool(np.array_equal(
+                     host_scores.view(np.uint32), result_scores.view(np.uint32))),
+                 "vs_reference_host_arrays": round(n / host_best / REFERENCE_PTS_S, 3) if
+                 args.workload == "plain-200k" else None,
+                 "note": "culof_amd.lof(numpy float32) best of 7: PCIe copies, allocation from the "
+                         "stream-ordered pool and the final synchronisation included"}
+    log(f"[bench] host-array path {host_path}")
+
     extra = {} if args.no_forms else forms_legs(args.seed, max(2, min(args.steps, 5)))
 
     out = {
@@ -359,6 +380,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,
         "sklearn": sklearn_leg,
+        "host_path": host_path,
         "kernels": kernels,
         "parity": parity,
         **extra,
