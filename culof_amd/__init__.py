@@ -156,8 +156,10 @@ class LOF:
 
     The transductive ``fit_predict`` workflow of
     :class:`sklearn.neighbors.LocalOutlierFactor` with the same sign
-    conventions (reference: src/culof/__init__.py:116-227). There is no
-    ``novelty`` mode, no ``get_params``/``set_params`` and no metric selection.
+    conventions (reference: src/culof/__init__.py:116-227). Beyond the
+    reference: ``get_params`` / ``set_params`` (so ``sklearn.base.clone``
+    works) and ``kneighbors`` of the fitted samples. There is no ``novelty``
+    mode and no metric selection (Euclidean only).
 
     Parameters
     ----------
@@ -181,7 +183,13 @@ class LOF:
         Threshold on :attr:`negative_outlier_factor_` used by :meth:`predict`.
     n_neighbors_ : int
         The value of ``n_neighbors`` actually used.
+    n_features_in_ : int
+        Number of features of the fitted data.
+    n_samples_fit_ : int
+        Number of fitted samples.
     """
+
+    _PARAMS = ("n_neighbors", "contamination", "normalize", "form", "ties")
 
     def __init__(
         self,
@@ -191,20 +199,45 @@ class LOF:
         form: str = "fp32",
         ties: str = "index",
     ) -> None:
-        if n_neighbors < 1:
-            raise ValueError(f"n_neighbors must be >= 1, got {n_neighbors}")
-        if contamination != "auto" and not 0.0 < float(contamination) <= 0.5:
-            raise ValueError(f"contamination must be 'auto' or in (0, 0.5], got {contamination!r}")
+        self._check_params(n_neighbors, contamination)
         self.n_neighbors = n_neighbors
         self.contamination = contamination
         self.normalize = normalize
         self.form = form
         self.ties = ties
 
+    @staticmethod
+    def _check_params(n_neighbors: int, contamination: str | float) -> None:
+        if n_neighbors < 1:
+            raise ValueError(f"n_neighbors must be >= 1, got {n_neighbors}")
+        if contamination != "auto" and not 0.0 < float(contamination) <= 0.5:
+            raise ValueError(f"contamination must be 'auto' or in (0, 0.5], got {contamination!r}")
+
+    def get_params(self, deep: bool = True) -> dict:
+        """The constructor parameters (scikit-learn's estimator protocol)."""
+        del deep  # no nested estimators
+        return {name: getattr(self, name) for name in self._PARAMS}
+
+    def set_params(self, **params: object) -> LOF:
+        """Set constructor parameters by name; returns ``self``. A refit is
+        needed for them to take effect on the fitted attributes."""
+        unknown = sorted(set(params) - set(self._PARAMS))
+        if unknown:
+            raise ValueError(f"invalid parameter(s) {unknown} for LOF; valid: {list(self._PARAMS)}")
+        merged = {**self.get_params(), **params}
+        self._check_params(merged["n_neighbors"], merged["contamination"])
+        for name, value in params.items():
+            setattr(self, name, value)
+        return self
+
     def fit(self, X: ArrayLike, y: object = None) -> LOF:
         """Compute the LOF of every sample in ``X``. Returns ``self``."""
         del y  # accepted for scikit-learn call compatibility
-        scores = lof(X, self.n_neighbors, self.normalize, self.form, self.ties)
+        arr = _validate(X, self.n_neighbors)
+        scores = lof(arr, self.n_neighbors, self.normalize, self.form, self.ties)
+        self._fit_X = arr
+        self.n_features_in_ = int(arr.shape[1])
+        self.n_samples_fit_ = int(arr.shape[0])
         self.n_neighbors_ = self.n_neighbors
         self.negative_outlier_factor_ = -scores
         if self.contamination == "auto":
@@ -231,6 +264,41 @@ class LOF:
     def predict(self) -> NDArray[np.intp]:
         """Label the fitted samples ``-1`` (outlier) or ``1`` (inlier)."""
         return np.where(self.score_samples() < self._check_offset(), -1, 1)
+
+    def kneighbors(self, X: None = None, n_neighbors: int | None = None,
+                   return_distance: bool = True):
+        """Neighbours of each fitted sample, itself excluded, nearest first.
+
+        Only ``X=None`` (the fitted samples) is supported. Returns
+        ``(distances, indices)`` of shape ``(n_samples_fit_, n_neighbors)``
+        (``indices`` alone when ``return_distance`` is false); distances are
+        float32 Euclidean distances in the space the LOF was computed in (after
+        z-scoring when ``normalize``), equal distances ordered by index. The
+        neighbour sets are the ones the LOF used (same tie rule and form).
+        """
+        self._check_fitted()
+        if X is not None:
+            raise ValueError("kneighbors supports only X=None (the fitted samples); "
+                             "there is no novelty mode")
+        k = self.n_neighbors_ if n_neighbors is None else int(n_neighbors)
+        if not 1 <= k <= self.n_samples_fit_ - 1:
+            raise ValueError(f"n_neighbors must satisfy 1 <= n_neighbors <= n_samples_fit_ - 1 "
+                             f"(got {k}, n_samples_fit_={self.n_samples_fit_})")
+        from .autograd import forward_state
+
+        state = forward_state(torch.from_numpy(self._fit_X).cuda(), k, self.normalize,
+                              self.form, self.ties)
+        idx = state.idx.cpu().numpy().T.astype(np.intp)
+        d2 = state.d2.cpu().numpy().T
+        # (distance, index) order: non-negative float32 bit patterns sort like
+        # the values, so one 64-bit key per entry orders both.
+        key = (d2.view(np.uint32).astype(np.uint64) << np.uint64(32)) | idx.astype(np.uint64)
+        order = np.argsort(key, axis=1)
+        idx = np.take_along_axis(idx, order, axis=1)
+        if not return_distance:
+            return idx
+        dist = np.sqrt(np.take_along_axis(d2, order, axis=1))
+        return dist, idx
 
     def _check_fitted(self) -> NDArray[np.float32]:
         if not hasattr(self, "negative_outlier_factor_"):
