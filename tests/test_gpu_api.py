@@ -131,6 +131,7 @@ def test_tile_rows_do_not_change_results(monkeypatch):
     monkeypatch.delenv("CULOF_TILE_ROWS")
 
 
+@pytest.mark.filterwarnings("ignore:Duplicate values")  # scikit-learn's own note on planted copies
 def test_duplicates_do_not_collapse():
     X = points(1000, 3, seed=5)
     X[500:530] = X[0]                                   # 31 copies of one point
