@@ -158,8 +158,10 @@ class LOF:
     :class:`sklearn.neighbors.LocalOutlierFactor` with the same sign
     conventions (reference: src/culof/__init__.py:116-227). Beyond the
     reference: ``get_params`` / ``set_params`` (so ``sklearn.base.clone``
-    works) and ``kneighbors`` of the fitted samples. There is no ``novelty``
-    mode and no metric selection (Euclidean only).
+    works), ``kneighbors`` of the fitted samples, and scikit-learn's
+    ``novelty=True`` mode (``score_samples(X)``, ``decision_function(X)`` and
+    ``predict(X)`` on new points; ``fit_predict`` is then unavailable, as in
+    scikit-learn). No metric selection (Euclidean only).
 
     Parameters
     ----------
@@ -173,7 +175,11 @@ class LOF:
     form : {"fp32", "bf16"}, default="fp32"
         Precision form of the distance screen; both give the same scores.
     ties : {"index", "fp64"}, default="index"
-        Tie rule at the k-th neighbour; see :func:`lof`.
+        Tie rule at the k-th neighbour; see :func:`lof`. New points in
+        novelty mode always take the index rule.
+    novelty : bool, default=False
+        Score new points against the fitted samples instead of the fitted
+        samples themselves (scikit-learn's semantics).
 
     Attributes
     ----------
@@ -189,7 +195,7 @@ class LOF:
         Number of fitted samples.
     """
 
-    _PARAMS = ("n_neighbors", "contamination", "normalize", "form", "ties")
+    _PARAMS = ("n_neighbors", "contamination", "normalize", "form", "ties", "novelty")
 
     def __init__(
         self,
@@ -198,6 +204,7 @@ class LOF:
         normalize: bool = False,
         form: str = "fp32",
         ties: str = "index",
+        novelty: bool = False,
     ) -> None:
         self._check_params(n_neighbors, contamination)
         self.n_neighbors = n_neighbors
@@ -205,6 +212,7 @@ class LOF:
         self.normalize = normalize
         self.form = form
         self.ties = ties
+        self.novelty = novelty
 
     @staticmethod
     def _check_params(n_neighbors: int, contamination: str | float) -> None:
@@ -251,19 +259,63 @@ class LOF:
     def fit_predict(self, X: ArrayLike, y: object = None) -> NDArray[np.intp]:
         """Fit and label each sample: ``-1`` for outliers, ``1`` for inliers."""
         del y
+        if self.novelty:
+            raise AttributeError("fit_predict is not available when novelty=True; "
+                                 "use fit(X).predict(X_new) on new data")
         return self.fit(X).predict()
 
-    def score_samples(self) -> NDArray[np.float32]:
-        """Negated LOF of the fitted data; the lower, the more abnormal.
+    def score_samples(self, X: ArrayLike | None = None) -> NDArray[np.float32]:
+        """Negated LOF; the lower, the more abnormal.
 
-        Takes no ``X``: scoring new points against each other would not be
-        scikit-learn's novelty mode, so the argument does not exist.
+        Without novelty: of the fitted data, and no ``X`` is taken (scoring new
+        points against each other would not be scikit-learn's novelty mode).
+        With ``novelty=True``: of the new points ``X`` against the fitted data.
         """
-        return self._check_fitted()
+        if not self.novelty:
+            if X is not None:
+                raise TypeError("score_samples takes no X unless novelty=True "
+                                "(it scores the fitted samples)")
+            return self._check_fitted()
+        if X is None:
+            raise TypeError("score_samples(X) needs the new points when novelty=True")
+        self._check_fitted()
+        return -self._novelty_lof(X)
 
-    def predict(self) -> NDArray[np.intp]:
-        """Label the fitted samples ``-1`` (outlier) or ``1`` (inlier)."""
-        return np.where(self.score_samples() < self._check_offset(), -1, 1)
+    def decision_function(self, X: ArrayLike) -> NDArray[np.float32]:
+        """``score_samples(X) - offset_`` for new points (novelty mode only):
+        negative for outliers."""
+        if not self.novelty:
+            raise AttributeError("decision_function is only available when novelty=True")
+        return self.score_samples(X) - np.float32(self._check_offset())
+
+    def predict(self, X: ArrayLike | None = None) -> NDArray[np.intp]:
+        """Label ``-1`` (outlier) or ``1`` (inlier): the fitted samples, or
+        with ``novelty=True`` the new points ``X``."""
+        if self.novelty:
+            return np.where(self.decision_function(X) < 0, -1, 1)
+        return np.where(self.score_samples(X) < self._check_offset(), -1, 1)
+
+    def _novelty_lof(self, X: ArrayLike) -> NDArray[np.float32]:
+        q = np.ascontiguousarray(X, dtype=np.float32)
+        if q.ndim != 2 or q.shape[1] != self.n_features_in_:
+            raise ValueError(f"X has shape {q.shape}; expected (n, {self.n_features_in_})")
+        if not np.isfinite(q).all():
+            raise ValueError("X contains NaN or infinity")
+        _check_representable(np.concatenate([self._fit_X, q]), self.n_features_in_)
+        from . import _native
+        from .autograd import form_code
+
+        fit = torch.from_numpy(self._fit_X).cuda()
+        new = torch.from_numpy(q).cuda()
+        out = torch.empty(q.shape[0], dtype=torch.float32, device=fit.device)
+        _native.check(_native.lib().culof_lof_novelty_device(
+            fit.data_ptr(), fit.shape[0], fit.shape[1], new.data_ptr(), q.shape[0], self.n_neighbors_,
+            int(bool(self.normalize)), form_code(self.form, self.ties), None, out.data_ptr(),
+            _native.current_stream()))
+        scores = out.cpu().numpy()
+        if not np.isfinite(scores).all():
+            raise ValueError("LOF produced non-finite scores for the new points; rescale the data")
+        return scores
 
     def kneighbors(self, X: None = None, n_neighbors: int | None = None,
                    return_distance: bool = True):
@@ -278,8 +330,7 @@ class LOF:
         """
         self._check_fitted()
         if X is not None:
-            raise ValueError("kneighbors supports only X=None (the fitted samples); "
-                             "there is no novelty mode")
+            raise ValueError("kneighbors supports only X=None (the fitted samples)")
         k = self.n_neighbors_ if n_neighbors is None else int(n_neighbors)
         if not 1 <= k <= self.n_samples_fit_ - 1:
             raise ValueError(f"n_neighbors must satisfy 1 <= n_neighbors <= n_samples_fit_ - 1 "
@@ -314,6 +365,7 @@ class LOF:
     def __repr__(self) -> str:
         form = "" if self.form == "fp32" else f", form={self.form!r}"
         form += "" if self.ties == "index" else f", ties={self.ties!r}"
+        form += ", novelty=True" if self.novelty else ""
         return (
             f"LOF(n_neighbors={self.n_neighbors}, "
             f"contamination={self.contamination!r}, normalize={self.normalize}{form})"

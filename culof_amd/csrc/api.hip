@@ -182,6 +182,7 @@ struct Prepared {
     const float* xc = nullptr;  // centred (and scaled) rows for the distance screen
     const float* norms = nullptr;
     const int* guard = nullptr;
+    const double* work = nullptr;  // center_and_norms' work: partials, mean, inv_std, scaled
 };
 
 Prepared prepare(const float* points, int n, int d, bool normalize, int form, Scratch& scratch,
@@ -201,6 +202,7 @@ Prepared prepare(const float* points, int n, int d, bool normalize, int form, Sc
     p.xc = xc;
     p.norms = norms;
     p.guard = guard;
+    p.work = work;
     return p;
 }
 
@@ -330,6 +332,45 @@ void forward(const float* points, int n, int d, int k, bool normalize, int form,
     compute_scores(nbr_idx, lrd, n, k, scores, s);
 }
 
+// New points scored against a fitted set (scikit-learn's novelty mode,
+// LocalOutlierFactor(novelty=True).score_samples): the forward on `points`
+// gives the fitted k-distances and densities; each query's k nearest fitted
+// points (exact FP32 keys on the same key rows -- the query rows themselves,
+// or z-scored with the fitted mean and scale -- none excluded, the
+// reference's tie rule and slot order) give its reach distances, density and
+// LOF. scores_fit (optional) receives the fitted points' own scores.
+void novelty(const float* points, int n, int d, const float* queries, int m, int k, bool normalize,
+             int form, float* scores_fit, float* scores_new, hipStream_t s) {
+    if (screen_form(form) < 0) throw std::invalid_argument("unknown precision form");
+    int device = 0;
+    CULOF_CHECK(hipGetDevice(&device));
+    enable_pool_reuse(device);
+    Scratch scratch(s);
+    const size_t nk = static_cast<size_t>(n) * k;
+    int* idx = scratch.get<int>(nk);
+    float* d2 = scratch.get<float>(nk);
+    float* kd = scratch.get<float>(static_cast<size_t>(n));
+    float* lrd = scratch.get<float>(static_cast<size_t>(n));
+    const Prepared p = prepare(points, n, d, normalize, screen_form(form), scratch, s);
+    knn_rows(p, n, d, k, form, 0, n, 0, idx, d2, kd, scratch, s);
+    compute_lrd(idx, d2, kd, n, k, lrd, s);
+    if (scores_fit) compute_scores(idx, lrd, n, k, scores_fit, s);
+    if (m <= 0) return;
+    const size_t cells = static_cast<size_t>(m) * p.stride;
+    float* qc = scratch.get<float>(cells);
+    float* qnorms = scratch.get<float>(static_cast<size_t>(m));
+    float* qcopy = normalize ? nullptr : scratch.get<float>(cells);
+    apply_rows(queries, m, d, p.work, qc, p.stride, qnorms, qcopy, s);
+    const size_t mk = static_cast<size_t>(m) * k;
+    int* qidx = scratch.get<int>(mk);
+    float* qd2 = scratch.get<float>(mk);
+    float* qkd = scratch.get<float>(static_cast<size_t>(m));
+    float* qlrd = scratch.get<float>(static_cast<size_t>(m));
+    radix_query_knn(normalize ? qc : qcopy, p.xs, p.stride, n, d, k, m, qidx, qd2, qkd, s);
+    compute_lrd(qidx, qd2, kd, m, k, qlrd, s);  // kd: the fitted k-distances, gathered by index
+    compute_query_scores(qidx, lrd, qlrd, m, k, scores_new, s);
+}
+
 std::vector<float> lof_host(const float* points, int n_points, int n_dims, int k, bool normalize) {
     validate(points, n_points, n_dims, k);
     require_device();
@@ -450,6 +491,20 @@ int culof_lof_device(const float* points, int n_points, int n_dims, int k, int n
         require_device();
         forward(points, n_points, n_dims, k, normalize != 0, form, scores, nbr_idx, nbr_d2,
                 kdist2, lrd, as_stream(stream));
+    });
+}
+
+int culof_lof_novelty_device(const float* points, int n_points, int n_dims, const float* queries,
+                             int n_queries, int k, int normalize, int form, float* scores_fit,
+                             float* scores_new, void* stream) {
+    return guarded([&] {
+        validate(points, n_points, n_dims, k);
+        if (n_queries < 0) throw std::invalid_argument("n_queries must be >= 0");
+        if (n_queries > 0 && (!queries || !scores_new))
+            throw std::invalid_argument("queries and scores_new must not be null");
+        require_device();
+        novelty(points, n_points, n_dims, queries, n_queries, k, normalize != 0, form, scores_fit,
+                scores_new, as_stream(stream));
     });
 }
 

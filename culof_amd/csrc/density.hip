@@ -32,12 +32,14 @@ lrd_kernel(const int* __restrict__ nbr_idx, const float* __restrict__ nbr_d2,
     lrd[i] = static_cast<float>(1.0 / (sum / k + kLrdEpsilon));
 }
 
+// lrd: the neighbours' densities; lrd_self: row i's own (the same array for
+// the fitted points, the queries' densities when scoring new points).
 __global__ void __launch_bounds__(kBlock)
-score_kernel(const int* __restrict__ nbr_idx, const float* __restrict__ lrd, int n, int k,
-             float* __restrict__ scores) {
+score_kernel(const int* __restrict__ nbr_idx, const float* __restrict__ lrd,
+             const float* __restrict__ lrd_self, int n, int k, float* __restrict__ scores) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double self = static_cast<double>(lrd[i]);
+    const double self = static_cast<double>(lrd_self[i]);
     double sum = 0.0;
     for (int t = 0; t < k; ++t) {
         sum += static_cast<double>(lrd[nbr_idx[static_cast<size_t>(t) * n + i]]) / self;
@@ -58,8 +60,17 @@ void compute_lrd(const int* nbr_idx, const float* nbr_d2, const float* kdist2, i
 void compute_scores(const int* nbr_idx, const float* lrd, int n_points, int k, float* scores,
                     hipStream_t stream) {
     ProfileScope prof("score", stream);
-    score_kernel<<<(n_points + kBlock - 1) / kBlock, kBlock, 0, stream>>>(nbr_idx, lrd, n_points,
+    score_kernel<<<(n_points + kBlock - 1) / kBlock, kBlock, 0, stream>>>(nbr_idx, lrd, lrd, n_points,
                                                                             k, scores);
+    CULOF_CHECK(hipGetLastError());
+}
+
+void compute_query_scores(const int* nbr_idx, const float* lrd_fit, const float* lrd_query,
+                          int n_queries, int k, float* scores, hipStream_t stream) {
+    ProfileScope prof("score", stream);
+    if (n_queries <= 0) return;
+    score_kernel<<<(n_queries + kBlock - 1) / kBlock, kBlock, 0, stream>>>(nbr_idx, lrd_fit, lrd_query,
+                                                                             n_queries, k, scores);
     CULOF_CHECK(hipGetLastError());
 }
 

@@ -65,6 +65,10 @@ private:
 void center_and_norms(const float* points, int n_points, int n_dims, bool scale, float* centred,
                       int stride, float* norms, double* work, float* copy, int* guard,
                       hipStream_t stream);
+// The same transform, with the mean and inv_std center_and_norms left in
+// `work`, applied to other rows (new points scored against a fitted set).
+void apply_rows(const float* rows, int n_rows, int n_dims, const double* work, float* out,
+                int stride, float* norms, float* copy, hipStream_t stream);
 
 // Selection keys are difference-form FP32 squared distances
 // e(i, j) = sum_t (a_t - b_t)^2 over the key rows (the input rows, or the
@@ -111,6 +115,11 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
 void radix_knn(const float* keyrows, int stride, int n_points, int n_dims, int k, int row0,
                int rows, const int* row_list, const int* row_count, int* nbr_idx, float* nbr_d2,
                float* kdist2, hipStream_t stream);
+// Query mode: rows of `qrows` (another row set, same stride) against all
+// n_points key rows, nothing excluded; outputs with stride n_queries.
+void radix_query_knn(const float* qrows, const float* keyrows, int stride, int n_points, int n_dims,
+                     int k, int n_queries, int* nbr_idx, float* nbr_d2, float* kdist2,
+                     hipStream_t stream);
 // Stored mode: keys row-major (rows x n_cols); outputs with stride `rows`.
 void radix_select_rows(const float* keys, int rows, int n_cols, int k, int* nbr_idx,
                        float* nbr_d2, float* kdist2, hipStream_t stream);
@@ -179,6 +188,10 @@ void compute_lrd(const int* nbr_idx, const float* nbr_d2, const float* kdist2, i
                  int k, float* lrd, hipStream_t stream);
 void compute_scores(const int* nbr_idx, const float* lrd, int n_points, int k, float* scores,
                     hipStream_t stream);
+// LOF of new points: mean of lrd_fit over each query's fitted neighbours
+// divided by the query's own lrd_query (scikit-learn's novelty score_samples).
+void compute_query_scores(const int* nbr_idx, const float* lrd_fit, const float* lrd_query,
+                          int n_queries, int k, float* scores, hipStream_t stream);
 
 // Same arithmetic as the reference's choose_tile_rows (src/cuda/pipeline.cu:33-38).
 int choose_tile_rows(int n_points, size_t budget_bytes);

@@ -153,5 +153,15 @@ void center_and_norms(const float* points, int n_points, int n_dims, bool scale,
     }
 }
 
+void apply_rows(const float* rows, int n_rows, int n_dims, const double* work, float* out,
+                int stride, float* norms, float* copy, hipStream_t stream) {
+    const double* mean = work + static_cast<size_t>(kLanes) * n_dims;
+    const double* inv_std = mean + n_dims;
+    if (n_rows <= 0) return;
+    center_apply_kernel<<<(n_rows + 255) / 256, 256, 0, stream>>>(rows, n_rows, n_dims, mean, inv_std, out,
+                                                                  stride, norms, copy, nullptr);
+    CULOF_CHECK(hipGetLastError());
+}
+
 }  // namespace detail
 }  // namespace culof

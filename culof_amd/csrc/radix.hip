@@ -216,7 +216,36 @@ radix_recompute_kernel(const float* __restrict__ xs, int stride, int d, int n, i
     }
 }
 
+// Queries from another row set: row r of `qrows` against every row of
+// `xs` (both at row stride `stride`), nothing excluded. Outputs with stride
+// n_queries. Grid-stride over the queries.
+__global__ void __launch_bounds__(kBlock)
+radix_query_kernel(const float* __restrict__ qrows, const float* __restrict__ xs, int stride, int d,
+                   int n, int k, int n_queries, int* __restrict__ nbr_idx, float* __restrict__ nbr_d2,
+                   float* __restrict__ kdist2) {
+    extern __shared__ float s_q[];  // stride floats
+    __shared__ RadixShared sh;
+    for (int r = blockIdx.x; r < n_queries; r += gridDim.x) {
+        __syncthreads();  // s_q reuse across rows
+        for (int t = threadIdx.x; t < stride; t += kBlock) s_q[t] = qrows[static_cast<size_t>(r) * stride + t];
+        __syncthreads();
+        const RecomputedKeys key{s_q, xs, stride, d, -1};
+        select_row(key, sh, n, k, r, n_queries, nbr_idx, nbr_d2, kdist2);
+    }
+}
+
 }  // namespace
+
+void radix_query_knn(const float* qrows, const float* keyrows, int stride, int n_points, int n_dims,
+                     int k, int n_queries, int* nbr_idx, float* nbr_d2, float* kdist2,
+                     hipStream_t stream) {
+    ProfileScope prof("radix_knn", stream);
+    if (n_queries <= 0) return;
+    const int grid = std::min(n_queries, 2048);
+    radix_query_kernel<<<grid, kBlock, sizeof(float) * stride, stream>>>(
+        qrows, keyrows, stride, n_dims, n_points, k, n_queries, nbr_idx, nbr_d2, kdist2);
+    CULOF_CHECK(hipGetLastError());
+}
 
 void radix_knn(const float* keyrows, int stride, int n_points, int n_dims, int k, int row0,
                int rows, const int* row_list, const int* row_count, int* nbr_idx, float* nbr_d2,

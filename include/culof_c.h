@@ -82,6 +82,20 @@ int culof_lof_device(const float* points, int n_points, int n_dims, int k, int n
                      int form, float* scores, int* nbr_idx, float* nbr_d2, float* kdist2,
                      float* lrd, void* stream);
 
+/* New points scored against a fitted set. No counterpart in the reference
+ * (its estimator has no novelty mode, README.md:29-34); this is scikit-learn's
+ * LocalOutlierFactor(novelty=True).fit(points).score_samples(queries) with the
+ * sign of culof_lof: scores_new[r] (n_queries floats) is the LOF of query row r
+ * against the fitted rows -- its k nearest fitted rows (none excluded; exact
+ * FP32 keys, the reference's tie rule), their k-distances and densities from
+ * the forward on `points`. scores_fit (optional, n_points floats) receives
+ * the fitted rows' own LOF. All pointers are device pointers; `form` selects
+ * the fitted forward's screen and tie rule (the queries use the index rule).
+ * For callers that score held-out data. Never synchronises. */
+int culof_lof_novelty_device(const float* points, int n_points, int n_dims, const float* queries,
+                             int n_queries, int k, int normalize, int form, float* scores_fit,
+                             float* scores_new, void* stream);
+
 /* Backward of the device-resident forward: grad_points (n_points x n_dims,
  * device) = d(sum_i grad_scores[i] * LOF_i) / d points, with the kNN sets and
  * the reach-distance max() branches fixed from a forward call on the same

@@ -63,6 +63,8 @@ def _load():
     lib.oracle_lof.restype = c_int
     lib.oracle_lof_fp64.argtypes = [_f32p, c_int, c_int, c_int, c_int, _f64p, c_void, c_void]
     lib.oracle_lof_fp64.restype = c_int
+    lib.oracle_lof_novelty.argtypes = [_f32p, c_int, c_int, _f32p, c_int, c_int, c_int, _f32p, c_void, c_void]
+    lib.oracle_lof_novelty.restype = c_int
     _lib = lib
     return lib
 
@@ -195,4 +197,27 @@ def lof_fp64(X, k: int = 20, mutation: int = 0, full: bool = False):
         raise ValueError("invalid arguments to oracle_lof_fp64")
     if full:
         return {"scores": out, "idx": idx, "kdist": kd}
+    return out
+
+
+def lof_novelty(X, Q, k: int = 20, normalize: bool = False, full: bool = False):
+    """LOF of new rows ``Q`` against the fitted rows ``X`` (scikit-learn's
+    novelty score_samples with culof's sign), in the forward's FP32/FP64 mix.
+    With ``full=True`` also returns each query's neighbour indices and keys
+    (row-major (m, k), slot order)."""
+    X = _as_f32(X)
+    Q = _as_f32(Q)
+    n, d = X.shape
+    m = Q.shape[0]
+    if Q.ndim != 2 or Q.shape[1] != d:
+        raise ValueError("queries must have the fitted rows' feature count")
+    out = np.empty(m, np.float32)
+    idx = np.empty((m, k), np.int32)
+    d2 = np.empty((m, k), np.float32)
+    rc = _load().oracle_lof_novelty(X, n, d, Q, m, int(k), int(normalize), out, idx.ctypes.data,
+                                    d2.ctypes.data)
+    if rc != 0:
+        raise ValueError("invalid arguments to oracle_lof_novelty")
+    if full:
+        return {"scores": out, "idx": idx, "d2": d2}
     return out

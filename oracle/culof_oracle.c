@@ -435,6 +435,63 @@ int oracle_lof(const float* x, int n, int d, int k, int normalize, int mutation,
 }
 
 /* ---------------------------------------------------------------------------
+ * New points against a fitted set (no reference code: scikit-learn's novelty
+ * mode, sklearn/neighbors/_lof.py score_samples, restated in the FP32/FP64 mix
+ * of the forward above). The forward on x gives the fitted k-distances and
+ * densities; query row r is keyed against every fitted key row (the raw rows,
+ * or z-scored with the fitted mean and scale), nothing excluded, selected by
+ * the same rule and slot order, then
+ *   lrd_q = 1 / (mean_t sqrt(max(kd[j_t], e_t)) + 1e-10),
+ *   lof_q = mean_t lrd[j_t] / lrd_q,
+ * both accumulated in FP64 in slot order. idx_out / d2_out (m x k, row-major)
+ * are optional.
+ * ------------------------------------------------------------------------- */
+
+int oracle_lof_novelty(const float* x, int n, int d, const float* q, int m, int k, int normalize,
+                       float* scores_out, int* idx_out, float* d2_out) {
+    if (!x || !q || n < 2 || d < 1 || k < 1 || k >= n || m < 0) return 1;
+    float* xc = (float*)malloc(sizeof(float) * (size_t)n * d);
+    float* norms = (float*)malloc(sizeof(float) * (size_t)n);
+    double* mean = (double*)malloc(sizeof(double) * (size_t)d);
+    double* inv_std = (double*)malloc(sizeof(double) * (size_t)d);
+    float* fit_scores = (float*)malloc(sizeof(float) * (size_t)n);
+    float* kd = (float*)malloc(sizeof(float) * (size_t)n);
+    float* lrd = (float*)malloc(sizeof(float) * (size_t)n);
+    float* qs = (float*)malloc(sizeof(float) * (size_t)(m > 0 ? m : 1) * d);
+    float* keys = (float*)malloc(sizeof(float) * (size_t)n);
+    uint32_t* scratch = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
+    int* idx = (int*)malloc(sizeof(int) * (size_t)k);
+    float* d2 = (float*)malloc(sizeof(float) * (size_t)k);
+
+    oracle_lof(x, n, d, k, normalize, 0, fit_scores, NULL, NULL, kd, lrd);
+    oracle_center(x, n, d, normalize, xc, norms, mean, inv_std);
+    for (int r = 0; r < m; ++r) {
+        for (int t = 0; t < d; ++t) {
+            const float v = q[(size_t)r * d + t];
+            qs[(size_t)r * d + t] = normalize ? (float)(((double)v - mean[t]) * inv_std[t]) : v;
+        }
+    }
+    const float* xs = normalize ? xc : x;
+    for (int r = 0; r < m; ++r) {
+        const float* qr = qs + (size_t)r * d;
+        for (int j = 0; j < n; ++j) keys[j] = key_of(qr, xs + (size_t)j * d, d);
+        float kth;
+        oracle_select_row(keys, n, k, scratch, idx, d2, &kth);
+        double reach = 0.0;
+        for (int t = 0; t < k; ++t) reach += sqrt((double)fmaxf(kd[idx[t]], d2[t]));
+        const float lrd_q = (float)(1.0 / (reach / k + 1e-10));
+        double ratio = 0.0;
+        for (int t = 0; t < k; ++t) ratio += (double)lrd[idx[t]] / (double)lrd_q;
+        scores_out[r] = (float)(ratio / k);
+        if (idx_out) memcpy(idx_out + (size_t)r * k, idx, sizeof(int) * (size_t)k);
+        if (d2_out) memcpy(d2_out + (size_t)r * k, d2, sizeof(float) * (size_t)k);
+    }
+    free(xc); free(norms); free(mean); free(inv_std); free(fit_scores); free(kd); free(lrd);
+    free(qs); free(keys); free(scratch); free(idx); free(d2);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------
  * FP64 brute-force LOF: tests/cpp/test_culof.cu:39-80 restated. Distances in
  * double from the uncentred input, ties ordered by (distance, index), the
  * k-distance is the distance to the k-th neighbour in that order.

@@ -48,6 +48,23 @@ def forward(X, k, normalize=False, form=_native.FORM_PLAIN):
             "kdist2": kd.cpu().numpy(), "lrd": lrd.cpu().numpy()}
 
 
+def novelty(X, Q, k, normalize=False, form=_native.FORM_PLAIN):
+    """(scores of the new rows Q, scores of the fitted rows X) via culof_lof_novelty_device."""
+    X = np.ascontiguousarray(X, np.float32)
+    Q = np.ascontiguousarray(Q, np.float32).reshape(-1, X.shape[1])
+    n, d = X.shape
+    m = Q.shape[0]
+    x = _dev(X)
+    q = _dev(Q) if m else None
+    fit = _empty(n, torch.float32)
+    new = _empty(max(m, 1), torch.float32)
+    _native.check(_native.lib().culof_lof_novelty_device(
+        x.data_ptr(), n, d, q.data_ptr() if m else None, m, k, int(normalize), form, fit.data_ptr(),
+        new.data_ptr(), _native.current_stream()))
+    _sync()
+    return new.cpu().numpy()[:m], fit.cpu().numpy()
+
+
 def center(X, normalize=False):
     X = np.ascontiguousarray(X, np.float32)
     n, d = X.shape

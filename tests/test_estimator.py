@@ -15,7 +15,8 @@ from culof_amd import LOF
 def test_params_round_trip_and_clone():
     m = LOF(n_neighbors=7, contamination=0.1, normalize=True, form="bf16", ties="fp64")
     p = m.get_params()
-    assert p == {"n_neighbors": 7, "contamination": 0.1, "normalize": True, "form": "bf16", "ties": "fp64"}
+    assert p == {"n_neighbors": 7, "contamination": 0.1, "normalize": True, "form": "bf16", "ties": "fp64",
+                 "novelty": False}
     c = clone(m)
     assert c is not m and c.get_params() == p
     assert m.set_params(n_neighbors=9, contamination="auto") is m
@@ -56,3 +57,16 @@ def test_kneighbors_matches_exact_search(form):
         m.kneighbors(X)
     with pytest.raises(ValueError, match="n_neighbors"):
         m.kneighbors(n_neighbors=3000)
+
+
+def test_novelty_argument_contract():
+    m = LOF(novelty=True)
+    assert m.get_params()["novelty"] is True and "novelty=True" in repr(m)
+    with pytest.raises(AttributeError, match="novelty"):
+        m.fit_predict(np.zeros((10, 2), np.float32))
+    with pytest.raises(TypeError, match="new points"):
+        m.score_samples()
+    with pytest.raises(AttributeError, match="novelty"):
+        LOF().decision_function(np.zeros((3, 2), np.float32))
+    with pytest.raises(TypeError):
+        LOF().score_samples(np.zeros((3, 2), np.float32))

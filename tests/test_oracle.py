@@ -321,3 +321,39 @@ def test_split_bf16_screen_bound_holds(name):
             worst_total = max(worst_total, float((err[sel] / (c * scale[sel])).max(initial=0)))
     assert worst_drop <= 1.0, worst_drop
     assert worst_total <= 1.0, worst_total
+
+
+@pytest.mark.parametrize("normalize", [False, True])
+def test_novelty_oracle_matches_sklearn(normalize):
+    """oracle_lof_novelty against scikit-learn's LocalOutlierFactor(novelty=True)
+    in FP64 (same z-scoring when normalising): the FP32 keys leave ~1e-7."""
+    from sklearn.neighbors import LocalOutlierFactor
+
+    rng = np.random.default_rng(21)
+    X = rng.standard_normal((900, 6)).astype(np.float32) * np.float32(3.0)
+    Q = np.concatenate([rng.standard_normal((150, 6)) * 3.0, rng.uniform(-20, 20, (10, 6))]).astype(np.float32)
+    got = oracle.lof_novelty(X, Q, 15, normalize=normalize)
+    Xs, Qs = X.astype(np.float64), Q.astype(np.float64)
+    if normalize:
+        mu, sd = Xs.mean(0), Xs.std(0)
+        Xs, Qs = (Xs - mu) / sd, (Qs - mu) / sd
+    want = -LocalOutlierFactor(n_neighbors=15, novelty=True).fit(Xs).score_samples(Qs)
+    assert np.max(np.abs(got - want) / want) < 2e-6
+    assert set(np.argsort(-got)[:10]) == set(range(150, 160))  # the far queries score highest
+
+
+def test_novelty_oracle_of_fitted_rows_and_identities():
+    """Closed forms on a unit grid (k = 4). The centre of an interior cell has
+    its four corners at squared distance 0.5, each of whose k-distance is 1,
+    as is every reach distance of their own neighbours: both densities are
+    1 / (1 + 1e-10) and the LOF is exactly 1. A query equal to a fitted row
+    finds that row at distance 0 (nothing is excluded). A query far outside
+    scores far above 1."""
+    g = np.arange(6, dtype=np.float32)
+    X = np.stack(np.meshgrid(g, g), -1).reshape(-1, 2)
+    Q = np.array([[2.0, 2.0], [2.5, 2.5], [30.0, 30.0]], np.float32)
+    res = oracle.lof_novelty(X, Q, 4, full=True)
+    assert res["scores"][1] == np.float32(1.0)
+    assert sorted(res["idx"][1]) == [14, 15, 20, 21] and (res["d2"][1] == 0.5).all()
+    assert res["d2"][0].min() == 0.0 and 14 in res["idx"][0]
+    assert res["scores"][2] > 5.0
