@@ -337,7 +337,8 @@ void forward(const float* points, int n, int d, int k, bool normalize, int form,
 // gives the fitted k-distances and densities; each query's k nearest fitted
 // points (exact FP32 keys on the same key rows -- the query rows themselves,
 // or z-scored with the fitted mean and scale -- none excluded, the
-// reference's tie rule and slot order) give its reach distances, density and
+// reference's tie rule and slot order; the VALU sweep, or the radix select
+// where the sweep is not instantiated) give its reach distances, density and
 // LOF. scores_fit (optional) receives the fitted points' own scores.
 void novelty(const float* points, int n, int d, const float* queries, int m, int k, bool normalize,
              int form, float* scores_fit, float* scores_new, hipStream_t s) {
@@ -366,7 +367,19 @@ void novelty(const float* points, int n, int d, const float* queries, int m, int
     float* qd2 = scratch.get<float>(mk);
     float* qkd = scratch.get<float>(static_cast<size_t>(m));
     float* qlrd = scratch.get<float>(static_cast<size_t>(m));
-    radix_query_knn(normalize ? qc : qcopy, p.xs, p.stride, n, d, k, m, qidx, qd2, qkd, s);
+    const float* qkeys = normalize ? qc : qcopy;
+    if (p.stride == padded_dims(d) && sweep_supported(d, k)) {
+        // The VALU sweep with the queries' own rows; the rows it hands back
+        // (huge norms, fewer than k finite keys) take the exact radix select.
+        int* fb_count = scratch.get<int>(1);
+        int* fb_rows = scratch.get<int>(static_cast<size_t>(m));
+        CULOF_CHECK(hipMemsetAsync(fb_count, 0, sizeof(int), s));
+        sweep_query_knn(qkeys, qc, qnorms, m, p.xs, p.xc, p.stride, p.norms, p.guard, n, d, k, qidx, qd2,
+                        qkd, fb_rows, fb_count, s);
+        radix_query_knn(qkeys, p.xs, p.stride, n, d, k, m, fb_rows, fb_count, qidx, qd2, qkd, s);
+    } else {
+        radix_query_knn(qkeys, p.xs, p.stride, n, d, k, m, nullptr, nullptr, qidx, qd2, qkd, s);
+    }
     compute_lrd(qidx, qd2, kd, m, k, qlrd, s);  // kd: the fitted k-distances, gathered by index
     compute_query_scores(qidx, lrd, qlrd, m, k, scores_new, s);
 }

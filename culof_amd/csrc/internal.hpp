@@ -87,6 +87,13 @@ void sweep_knn(const float* keyrows, const float* centred, int stride, const flo
                const int* guard, int n_points, int n_dims, int k, int row0, int rows,
                int* nbr_idx, float* nbr_d2, float* kdist2, int* fallback_rows,
                int* fallback_count, hipStream_t stream);
+// The same sweep for rows of another set (qkeys / qcentred / qnorms, same
+// stride and transform) against all n_points rows, nothing excluded; outputs
+// with stride n_queries; rows the sweep cannot settle go to fallback_rows.
+void sweep_query_knn(const float* qkeys, const float* qcentred, const float* qnorms, int n_queries,
+                     const float* keyrows, const float* centred, int stride, const float* norms,
+                     const int* guard, int n_points, int n_dims, int k, int* nbr_idx, float* nbr_d2,
+                     float* kdist2, int* fallback_rows, int* fallback_count, hipStream_t stream);
 
 // ---- mfma.hip: matrix-core screened sweep (FP32 or BF16 screen) ----
 // Row stride the form's screen expects (BF16 needs multiples of 16).
@@ -116,10 +123,11 @@ void radix_knn(const float* keyrows, int stride, int n_points, int n_dims, int k
                int rows, const int* row_list, const int* row_count, int* nbr_idx, float* nbr_d2,
                float* kdist2, hipStream_t stream);
 // Query mode: rows of `qrows` (another row set, same stride) against all
-// n_points key rows, nothing excluded; outputs with stride n_queries.
+// n_points key rows, nothing excluded; outputs with stride n_queries. All
+// queries, or row_list[0 .. *row_count) (device) when row_count is given.
 void radix_query_knn(const float* qrows, const float* keyrows, int stride, int n_points, int n_dims,
-                     int k, int n_queries, int* nbr_idx, float* nbr_d2, float* kdist2,
-                     hipStream_t stream);
+                     int k, int n_queries, const int* row_list, const int* row_count, int* nbr_idx,
+                     float* nbr_d2, float* kdist2, hipStream_t stream);
 // Stored mode: keys row-major (rows x n_cols); outputs with stride `rows`.
 void radix_select_rows(const float* keys, int rows, int n_cols, int k, int* nbr_idx,
                        float* nbr_d2, float* kdist2, hipStream_t stream);

@@ -218,14 +218,18 @@ radix_recompute_kernel(const float* __restrict__ xs, int stride, int d, int n, i
 
 // Queries from another row set: row r of `qrows` against every row of
 // `xs` (both at row stride `stride`), nothing excluded. Outputs with stride
-// n_queries. Grid-stride over the queries.
+// n_queries. Grid-stride over the queries [0, n_queries), or over
+// row_list[0 .. *count) when count is given.
 __global__ void __launch_bounds__(kBlock)
 radix_query_kernel(const float* __restrict__ qrows, const float* __restrict__ xs, int stride, int d,
-                   int n, int k, int n_queries, int* __restrict__ nbr_idx, float* __restrict__ nbr_d2,
+                   int n, int k, int n_queries, const int* __restrict__ row_list,
+                   const int* __restrict__ count, int* __restrict__ nbr_idx, float* __restrict__ nbr_d2,
                    float* __restrict__ kdist2) {
     extern __shared__ float s_q[];  // stride floats
     __shared__ RadixShared sh;
-    for (int r = blockIdx.x; r < n_queries; r += gridDim.x) {
+    const int total = count ? *count : n_queries;
+    for (int i = blockIdx.x; i < total; i += gridDim.x) {
+        const int r = row_list ? row_list[i] : i;
         __syncthreads();  // s_q reuse across rows
         for (int t = threadIdx.x; t < stride; t += kBlock) s_q[t] = qrows[static_cast<size_t>(r) * stride + t];
         __syncthreads();
@@ -237,13 +241,14 @@ radix_query_kernel(const float* __restrict__ qrows, const float* __restrict__ xs
 }  // namespace
 
 void radix_query_knn(const float* qrows, const float* keyrows, int stride, int n_points, int n_dims,
-                     int k, int n_queries, int* nbr_idx, float* nbr_d2, float* kdist2,
-                     hipStream_t stream) {
+                     int k, int n_queries, const int* row_list, const int* row_count, int* nbr_idx,
+                     float* nbr_d2, float* kdist2, hipStream_t stream) {
     ProfileScope prof("radix_knn", stream);
     if (n_queries <= 0) return;
     const int grid = std::min(n_queries, 2048);
     radix_query_kernel<<<grid, kBlock, sizeof(float) * stride, stream>>>(
-        qrows, keyrows, stride, n_dims, n_points, k, n_queries, nbr_idx, nbr_d2, kdist2);
+        qrows, keyrows, stride, n_dims, n_points, k, n_queries, row_list, row_count, nbr_idx, nbr_d2,
+        kdist2);
     CULOF_CHECK(hipGetLastError());
 }
 

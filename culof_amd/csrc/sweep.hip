@@ -116,7 +116,8 @@ sweep_kernel(const float* __restrict__ xs, const float* __restrict__ xc,
              const float* __restrict__ norms, const int* __restrict__ guard, int n, int k,
              float screen_c, int row0, int rows, int* __restrict__ nbr_idx,
              float* __restrict__ nbr_d2, float* __restrict__ kdist2, int* __restrict__ fb_rows,
-             int* __restrict__ fb_count) {
+             int* __restrict__ fb_count, const float* __restrict__ qxs, const float* __restrict__ qxc,
+             const float* __restrict__ qnorms, int same_set, int out_stride) {
     using Shape = SweepShape<DP>;
     constexpr int CH = Shape::kChunk;
     constexpr int NV = Shape::kVec;
@@ -155,7 +156,7 @@ sweep_kernel(const float* __restrict__ xs, const float* __restrict__ xc,
     float qc[DP];
     float qs[Shape::kKeyRowInRegs ? DP : 1];
     {
-        const float4* src = reinterpret_cast<const float4*>(xc + static_cast<size_t>(qi) * DP);
+        const float4* src = reinterpret_cast<const float4*>(qxc + static_cast<size_t>(qi) * DP);
 #pragma unroll
         for (int v = 0; v < V4; ++v) {
             const float4 t = src[v];
@@ -165,7 +166,7 @@ sweep_kernel(const float* __restrict__ xs, const float* __restrict__ xc,
             qc[4 * v + 3] = t.w;
         }
         if constexpr (Shape::kKeyRowInRegs) {
-            const float4* srs = reinterpret_cast<const float4*>(xs + static_cast<size_t>(qi) * DP);
+            const float4* srs = reinterpret_cast<const float4*>(qxs + static_cast<size_t>(qi) * DP);
 #pragma unroll
             for (int v = 0; v < V4; ++v) {
                 const float4 t = srs[v];
@@ -176,7 +177,7 @@ sweep_kernel(const float* __restrict__ xs, const float* __restrict__ xc,
             }
         }
     }
-    const float nq = norms[qi];
+    const float nq = qnorms[qi];
     const float cq = screen_c * nq;
     const float up = 1.0f + 4.0f * 5.9604645e-8f;  // rounds the screen threshold upward
 
@@ -288,7 +289,7 @@ sweep_kernel(const float* __restrict__ xs, const float* __restrict__ xc,
                         // Exact key: difference form over the key rows.
                         const int j = cur_base + c + g;
                         const float4* sv = s_cs + (c + g) * V4;
-                        const float4* qv = reinterpret_cast<const float4*>(xs + static_cast<size_t>(qi) * DP);
+                        const float4* qv = reinterpret_cast<const float4*>(qxs + static_cast<size_t>(qi) * DP);
                         float e = 0.0f;
 #pragma unroll
                         for (int v = 0; v < V4; ++v) {
@@ -308,7 +309,7 @@ sweep_kernel(const float* __restrict__ xs, const float* __restrict__ xc,
                             df = q4.w - t.w;
                             e = __builtin_fmaf(df, df, e);
                         }
-                        const uint32_t kb = (j == qi || e != e) ? kInfBits : __float_as_uint(e);
+                        const uint32_t kb = ((same_set && j == qi) || e != e) ? kInfBits : __float_as_uint(e);
                         const uint64_t p = pack(kb, static_cast<uint32_t>(j));
                         if (p < a[0]) {
                             s_ring[cnt * 64 + lane] = p;
@@ -365,7 +366,7 @@ sweep_kernel(const float* __restrict__ xs, const float* __restrict__ xc,
     // their slot is their position. The prefix is re-ranked by index.
     int below = 0;
     for (int t = 0; t < k; ++t) below += static_cast<uint32_t>(s_merge[t * 64 + lane] >> 32) < kth;
-    const size_t ns = static_cast<size_t>(n);
+    const size_t ns = static_cast<size_t>(out_stride);
     for (int e = 0; e < k; ++e) {
         const uint64_t w = s_merge[e * 64 + lane];
         const uint32_t idx = static_cast<uint32_t>(w);
@@ -394,6 +395,13 @@ struct SweepArgs {
     float* kdist2;
     int* fb_rows;
     int* fb_count;
+    // Query rows: the candidates themselves (same_set, the point itself
+    // excluded, outputs at stride n) or another row set (outputs at stride rows).
+    const float* qxs;
+    const float* qxc;
+    const float* qnorms;
+    int same_set;
+    int out_stride;
 };
 
 template <int DP, int KMAX>
@@ -401,7 +409,7 @@ void launch_sweep(const SweepArgs& a, hipStream_t stream) {
     const int grid = (a.rows + 63) / 64;
     sweep_kernel<DP, KMAX><<<grid, 64 * kWaves, 0, stream>>>(
         a.xs, a.xc, a.norms, a.guard, a.n, a.k, a.screen_c, a.row0, a.rows, a.nbr_idx, a.nbr_d2,
-        a.kdist2, a.fb_rows, a.fb_count);
+        a.kdist2, a.fb_rows, a.fb_count, a.qxs, a.qxc, a.qnorms, a.same_set, a.out_stride);
 }
 
 template <int DP>
@@ -460,8 +468,24 @@ void sweep_knn(const float* keyrows, const float* centred, int stride, const flo
     ProfileScope prof("sweep_knn", stream);
     const SweepArgs args{keyrows, centred, norms, guard, n_points, k,
                          static_cast<float>(n_dims + 16) * 5.9604645e-8f,  // (d + 16) 2^-24
-                         row0, rows, nbr_idx, nbr_d2, kdist2, fallback_rows, fallback_count};
+                         row0, rows, nbr_idx, nbr_d2, kdist2, fallback_rows, fallback_count,
+                         keyrows, centred, norms, 1, n_points};
     if (!dispatch(stride, k, &args, stream)) throw std::invalid_argument("sweep_knn: unsupported (n_dims, k)");
+    CULOF_CHECK(hipGetLastError());
+}
+
+void sweep_query_knn(const float* qkeys, const float* qcentred, const float* qnorms, int n_queries,
+                     const float* keyrows, const float* centred, int stride, const float* norms,
+                     const int* guard, int n_points, int n_dims, int k, int* nbr_idx, float* nbr_d2,
+                     float* kdist2, int* fallback_rows, int* fallback_count, hipStream_t stream) {
+    if (stride != padded_dims(n_dims)) throw std::invalid_argument("sweep_query_knn: row stride mismatch");
+    if (n_queries <= 0) return;
+    ProfileScope prof("sweep_knn", stream);
+    const SweepArgs args{keyrows, centred, norms, guard, n_points, k,
+                         static_cast<float>(n_dims + 16) * 5.9604645e-8f,  // (d + 16) 2^-24
+                         0, n_queries, nbr_idx, nbr_d2, kdist2, fallback_rows, fallback_count,
+                         qkeys, qcentred, qnorms, 0, n_queries};
+    if (!dispatch(stride, k, &args, stream)) throw std::invalid_argument("sweep_query_knn: unsupported (n_dims, k)");
     CULOF_CHECK(hipGetLastError());
 }
 
