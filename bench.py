@@ -26,6 +26,8 @@ torch.cuda.synchronize(). Prints one JSON line:
                uses), with its scores compared to the device path's
 * fwd_bwd      forward + hand-written backward of sum(w * LOF) at n=50k, d=32,
                k=20 (points/s of both passes, and the ratio to the forward)
+* novelty      200k new points scored against the headline workload's fit
+               (culof_lof_novelty_device, fit included), new points/s
 * forms        the BF16-screen form against the plain FP32 form at n=100k,
                d=128, k=50: points/s of each and whether the kNN sets, the
                k-distances and the scores are identical (and both forms on
@@ -190,6 +192,29 @@ def forms_legs(seed: int, steps: int) -> dict:
         "identical_scores": bool(torch.equal(res["fp32"].scores, res["bf16"].scores)),
     }
     log(f"[bench] forms (headline workload) {out['forms_headline']}")
+
+    # Novelty: 200k new points (jittered copies of fitted rows) scored against
+    # the headline workload's fit, through culof_lof_novelty_device; the fit is
+    # part of every call.
+    from culof_amd import _native
+
+    rng = np.random.default_rng(seed + 1)
+    Q = (X[rng.integers(0, n, n)] + rng.standard_normal((n, d)).astype(np.float32) * np.float32(0.1))
+    q = torch.from_numpy(np.ascontiguousarray(Q, dtype=np.float32)).cuda()
+    new = torch.empty(n, dtype=torch.float32, device="cuda")
+
+    def novelty():
+        _native.check(_native.lib().culof_lof_novelty_device(
+            x.data_ptr(), n, d, q.data_ptr(), n, k, 0, 0, None, new.data_ptr(), _native.current_stream()))
+
+    t = _time_device(novelty, steps)
+    out["novelty"] = {
+        "workload": "plain-200k fit, 200k new points", "n": n, "n_new": n, "d": d, "k": k,
+        "new_points_per_s": round(n / t, 2), "ms_per_call": round(t * 1e3, 3),
+        "scores_finite": bool(torch.isfinite(new).all()),
+        "note": "fit (forward on the fitted rows) and query kNN + scores in every call",
+    }
+    log(f"[bench] novelty {out['novelty']}")
     return out
 
 
