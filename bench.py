@@ -205,7 +205,8 @@ def forms_legs(seed: int, steps: int) -> dict:
 
     def novelty():
         _native.check(_native.lib().culof_lof_novelty_device(
-            x.data_ptr(), n, d, q.data_ptr(), n, k, 0, 0, None, new.data_ptr(), _native.current_stream()))
+            x.data_ptr(), n, d, q.data_ptr(), n, k, 0, 0, None, new.data_ptr(), None, None,
+            _native.current_stream()))
 
     t = _time_device(novelty, steps)
     out["novelty"] = {

@@ -279,7 +279,7 @@ class LOF:
         if X is None:
             raise TypeError("score_samples(X) needs the new points when novelty=True")
         self._check_fitted()
-        return -self._novelty_lof(X)
+        return -self._novelty(X, self.n_neighbors_)[0]
 
     def decision_function(self, X: ArrayLike) -> NDArray[np.float32]:
         """``score_samples(X) - offset_`` for new points (novelty mode only):
@@ -295,7 +295,9 @@ class LOF:
             return np.where(self.decision_function(X) < 0, -1, 1)
         return np.where(self.score_samples(X) < self._check_offset(), -1, 1)
 
-    def _novelty_lof(self, X: ArrayLike) -> NDArray[np.float32]:
+    def _novelty(self, X: ArrayLike, k: int, neighbours: bool = False):
+        """(LOF of the new rows X against the fitted rows, and with
+        ``neighbours`` their (k, m) neighbour indices and squared keys)."""
         q = np.ascontiguousarray(X, dtype=np.float32)
         if q.ndim != 2 or q.shape[1] != self.n_features_in_:
             raise ValueError(f"X has shape {q.shape}; expected (n, {self.n_features_in_})")
@@ -307,40 +309,49 @@ class LOF:
 
         fit = torch.from_numpy(self._fit_X).cuda()
         new = torch.from_numpy(q).cuda()
-        out = torch.empty(q.shape[0], dtype=torch.float32, device=fit.device)
+        m = q.shape[0]
+        out = torch.empty(m, dtype=torch.float32, device=fit.device)
+        idx = torch.empty((k, m), dtype=torch.int32, device=fit.device) if neighbours else None
+        d2 = torch.empty((k, m), dtype=torch.float32, device=fit.device) if neighbours else None
         _native.check(_native.lib().culof_lof_novelty_device(
-            fit.data_ptr(), fit.shape[0], fit.shape[1], new.data_ptr(), q.shape[0], self.n_neighbors_,
+            fit.data_ptr(), fit.shape[0], fit.shape[1], new.data_ptr(), m, k,
             int(bool(self.normalize)), form_code(self.form, self.ties), None, out.data_ptr(),
-            _native.current_stream()))
+            _native.ptr(idx), _native.ptr(d2), _native.current_stream()))
         scores = out.cpu().numpy()
         if not np.isfinite(scores).all():
             raise ValueError("LOF produced non-finite scores for the new points; rescale the data")
-        return scores
+        if not neighbours:
+            return scores, None, None
+        return scores, idx.cpu().numpy(), d2.cpu().numpy()
 
-    def kneighbors(self, X: None = None, n_neighbors: int | None = None,
+    def kneighbors(self, X: ArrayLike | None = None, n_neighbors: int | None = None,
                    return_distance: bool = True):
-        """Neighbours of each fitted sample, itself excluded, nearest first.
+        """Nearest fitted samples, nearest first.
 
-        Only ``X=None`` (the fitted samples) is supported. Returns
-        ``(distances, indices)`` of shape ``(n_samples_fit_, n_neighbors)``
-        (``indices`` alone when ``return_distance`` is false); distances are
-        float32 Euclidean distances in the space the LOF was computed in (after
-        z-scoring when ``normalize``), equal distances ordered by index. The
-        neighbour sets are the ones the LOF used (same tie rule and form).
+        With ``X=None``: of each fitted sample, itself excluded, the sets the
+        LOF used (same tie rule and form). With ``X``: of each new row, nothing
+        excluded (the index tie rule). Returns ``(distances, indices)`` of shape
+        ``(n_rows, n_neighbors)`` (``indices`` alone when ``return_distance`` is
+        false); distances are float32 Euclidean distances in the space the LOF
+        was computed in (after z-scoring when ``normalize``), equal distances
+        ordered by index.
         """
         self._check_fitted()
-        if X is not None:
-            raise ValueError("kneighbors supports only X=None (the fitted samples)")
         k = self.n_neighbors_ if n_neighbors is None else int(n_neighbors)
         if not 1 <= k <= self.n_samples_fit_ - 1:
             raise ValueError(f"n_neighbors must satisfy 1 <= n_neighbors <= n_samples_fit_ - 1 "
                              f"(got {k}, n_samples_fit_={self.n_samples_fit_})")
-        from .autograd import forward_state
+        if X is not None:
+            _, idx_t, d2_t = self._novelty(X, k, neighbours=True)
+            idx = idx_t.T.astype(np.intp)
+            d2 = np.ascontiguousarray(d2_t.T)
+        else:
+            from .autograd import forward_state
 
-        state = forward_state(torch.from_numpy(self._fit_X).cuda(), k, self.normalize,
-                              self.form, self.ties)
-        idx = state.idx.cpu().numpy().T.astype(np.intp)
-        d2 = state.d2.cpu().numpy().T
+            state = forward_state(torch.from_numpy(self._fit_X).cuda(), k, self.normalize,
+                                  self.form, self.ties)
+            idx = state.idx.cpu().numpy().T.astype(np.intp)
+            d2 = state.d2.cpu().numpy().T
         # (distance, index) order: non-negative float32 bit patterns sort like
         # the values, so one 64-bit key per entry orders both.
         key = (d2.view(np.uint32).astype(np.uint64) << np.uint64(32)) | idx.astype(np.uint64)

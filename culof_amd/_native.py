@@ -34,7 +34,7 @@ SIGNATURES = {
     "culof_lof": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp]),
     "culof_lof_device": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "culof_lof_novelty_device": (_c_int, [_vp, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp,
-                                          _vp]),
+                                          _vp, _vp, _vp]),
     "culof_lof_backward_device": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "culof_stage_center": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "culof_stage_knn": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp]),

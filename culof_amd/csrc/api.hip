@@ -341,7 +341,8 @@ void forward(const float* points, int n, int d, int k, bool normalize, int form,
 // where the sweep is not instantiated) give its reach distances, density and
 // LOF. scores_fit (optional) receives the fitted points' own scores.
 void novelty(const float* points, int n, int d, const float* queries, int m, int k, bool normalize,
-             int form, float* scores_fit, float* scores_new, hipStream_t s) {
+             int form, float* scores_fit, float* scores_new, int* nbr_idx_new, float* nbr_d2_new,
+             hipStream_t s) {
     if (screen_form(form) < 0) throw std::invalid_argument("unknown precision form");
     int device = 0;
     CULOF_CHECK(hipGetDevice(&device));
@@ -363,8 +364,8 @@ void novelty(const float* points, int n, int d, const float* queries, int m, int
     float* qcopy = normalize ? nullptr : scratch.get<float>(cells);
     apply_rows(queries, m, d, p.work, qc, p.stride, qnorms, qcopy, s);
     const size_t mk = static_cast<size_t>(m) * k;
-    int* qidx = scratch.get<int>(mk);
-    float* qd2 = scratch.get<float>(mk);
+    int* qidx = nbr_idx_new ? nbr_idx_new : scratch.get<int>(mk);
+    float* qd2 = nbr_d2_new ? nbr_d2_new : scratch.get<float>(mk);
     float* qkd = scratch.get<float>(static_cast<size_t>(m));
     float* qlrd = scratch.get<float>(static_cast<size_t>(m));
     const float* qkeys = normalize ? qc : qcopy;
@@ -509,7 +510,7 @@ int culof_lof_device(const float* points, int n_points, int n_dims, int k, int n
 
 int culof_lof_novelty_device(const float* points, int n_points, int n_dims, const float* queries,
                              int n_queries, int k, int normalize, int form, float* scores_fit,
-                             float* scores_new, void* stream) {
+                             float* scores_new, int* nbr_idx_new, float* nbr_d2_new, void* stream) {
     return guarded([&] {
         validate(points, n_points, n_dims, k);
         if (n_queries < 0) throw std::invalid_argument("n_queries must be >= 0");
@@ -517,7 +518,7 @@ int culof_lof_novelty_device(const float* points, int n_points, int n_dims, cons
             throw std::invalid_argument("queries and scores_new must not be null");
         require_device();
         novelty(points, n_points, n_dims, queries, n_queries, k, normalize != 0, form, scores_fit,
-                scores_new, as_stream(stream));
+                scores_new, nbr_idx_new, nbr_d2_new, as_stream(stream));
     });
 }
 

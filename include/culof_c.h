@@ -89,12 +89,15 @@ int culof_lof_device(const float* points, int n_points, int n_dims, int k, int n
  * against the fitted rows -- its k nearest fitted rows (none excluded; exact
  * FP32 keys, the reference's tie rule), their k-distances and densities from
  * the forward on `points`. scores_fit (optional, n_points floats) receives
- * the fitted rows' own LOF. All pointers are device pointers; `form` selects
- * the fitted forward's screen and tie rule (the queries use the index rule).
- * For callers that score held-out data. Never synchronises. */
+ * the fitted rows' own LOF; nbr_idx_new / nbr_d2_new (optional, k x n_queries,
+ * slot t of query r at [t * n_queries + r]) receive each query's fitted
+ * neighbours and squared-distance keys in the reference's slot order. All
+ * pointers are device pointers; `form` selects the fitted forward's screen and
+ * tie rule (the queries use the index rule). For callers that score held-out
+ * data. Never synchronises. */
 int culof_lof_novelty_device(const float* points, int n_points, int n_dims, const float* queries,
                              int n_queries, int k, int normalize, int form, float* scores_fit,
-                             float* scores_new, void* stream);
+                             float* scores_new, int* nbr_idx_new, float* nbr_d2_new, void* stream);
 
 /* Backward of the device-resident forward: grad_points (n_points x n_dims,
  * device) = d(sum_i grad_scores[i] * LOF_i) / d points, with the kNN sets and

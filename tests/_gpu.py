@@ -60,7 +60,7 @@ def novelty(X, Q, k, normalize=False, form=_native.FORM_PLAIN):
     new = _empty(max(m, 1), torch.float32)
     _native.check(_native.lib().culof_lof_novelty_device(
         x.data_ptr(), n, d, q.data_ptr() if m else None, m, k, int(normalize), form, fit.data_ptr(),
-        new.data_ptr(), _native.current_stream()))
+        new.data_ptr(), None, None, _native.current_stream()))
     _sync()
     return new.cpu().numpy()[:m], fit.cpu().numpy()
 

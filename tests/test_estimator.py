@@ -53,8 +53,13 @@ def test_kneighbors_matches_exact_search(form):
     np.testing.assert_array_equal(idx, ref_i[:, :12])          # generic data: no ties
     np.testing.assert_allclose(dist, ref_d[:, :12], rtol=2e-6)
     assert m.kneighbors(n_neighbors=3, return_distance=False).shape == (3000, 3)
-    with pytest.raises(ValueError, match="X=None"):
-        m.kneighbors(X)
+    # new rows: nothing excluded (a fitted row finds itself at distance 0)
+    Q = np.concatenate([rng.standard_normal((400, 16)).astype(np.float32), X[:5]])
+    qd, qi = m.kneighbors(Q, n_neighbors=7)
+    rd, ri = NearestNeighbors(n_neighbors=7).fit(X.astype(np.float64)).kneighbors(Q.astype(np.float64))
+    np.testing.assert_array_equal(qi, ri)
+    np.testing.assert_allclose(qd, rd, rtol=2e-6, atol=1e-6)
+    assert (qi[-5:, 0] == np.arange(5)).all() and (qd[-5:, 0] == 0).all()
     with pytest.raises(ValueError, match="n_neighbors"):
         m.kneighbors(n_neighbors=3000)
 
