@@ -67,3 +67,23 @@ def test_estimator_novelty_matches_sklearn():
     assert (ours.predict(Q)[-100:] == -1).mean() > 0.9       # the planted far queries
     with pytest.raises(AttributeError):
         ours.fit_predict(X)
+
+
+def test_novelty_guard_routes_queries_to_radix():
+    """A fitted norm past FLT_MAX / 16 sets the device guard: the sweep stands
+    down and every query takes the exact radix select from its row list (keys
+    stay finite at this scale); the scores still match the oracle bit for bit."""
+    from _gpu import novelty
+
+    rng = np.random.default_rng(92)
+    X = (rng.standard_normal((3000, 4)) * 9.5e17).astype(np.float32)
+    X[0] = np.array([5e18, 0.0, 0.0, 0.0], np.float32)      # norm 2.5e37 > FLT_MAX / 16
+    Q = (rng.standard_normal((200, 4)) * 9.5e17).astype(np.float32)
+    _native.profile_reset()
+    _native.profile_enable(True)
+    try:
+        new, _ = novelty(X, Q, 10)
+    finally:
+        _native.profile_enable(False)
+    assert _native.profile_read("radix_knn")[1] >= 2          # the fit's redo and the queries'
+    np.testing.assert_array_equal(new.view(np.uint32), oracle.lof_novelty(X, Q, 10).view(np.uint32))
