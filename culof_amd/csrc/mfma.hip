@@ -675,7 +675,9 @@ mfma_sweep_kernel(MfmaArgs p) {
     int cnt = 0;
     uint32_t* ring = reinterpret_cast<uint32_t*>(smem) + wave * kRing * 64;
 
-    auto flush = [&]() {
+    // Forced inline: an out-of-line call would put the list and the
+    // arguments it captures in scratch memory.
+    auto flush = [&]() __attribute__((always_inline)) {
         // Exact keys of the queued candidates, four at a time so their row
         // loads overlap, then sorted inserts.
         for (int r = 0; r < kRing; r += 4) {
