@@ -1819,6 +1819,10 @@ bool dispatch_k(const MfmaArgs* a, int k, hipStream_t stream) {
         if (a) launch_mfma<Op, DP, 32>(*a, stream);
         return true;
     }
+    if (k <= 48) {  // e.g. k + 1 = 33 of the FP64 tie-break at k = 32: two waves per SIMD
+        if (a) launch_mfma<Op, DP, 48>(*a, stream);
+        return true;
+    }
     if (k <= 64) {
         if (a) launch_mfma<Op, DP, 64>(*a, stream);
         return true;
