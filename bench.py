@@ -50,7 +50,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "LOF pts/s (n=200k,d=8,k=20) + fwd+bwd pts/s; bf16-screen vs fp32 at equal kNN"
 FP32_PEAK_TFLOPS = 157.3      # MI355X dense FP32 (matrix and packed vector)
-FP32_VALU_TFLOPS = 78.6       # non-packed FP32 VALU (packed FP32 ops are disabled in this build)
+FP32_VALU_TFLOPS = 78.6       # non-packed FP32 VALU (packed FP32 ops are disabled at compile time)
 HBM_PEAK_GBS = 8000.0
 REFERENCE_PTS_S = 1.42e5      # reference on RTX A6000 at n=200k, d=8, k=20 (its README.md:50)
 KERNELS = ("center_partials", "center_apply", "order", "seed", "heavy_rows", "screen_prep",

@@ -1,8 +1,9 @@
 """ctypes front end for the CPU oracle in ``culof_oracle.c``.
 
 Test infrastructure only: imported by ``tests/``, by ``__graft_entry__.smoke``
-and by the CPU-baseline leg of ``bench.py``, always as the checker. The
-product package ``culof_amd`` never imports it.
+and by the CPU-baseline leg of ``bench.py``, always to check or time the
+GPU path against, never in its place. The product package ``culof_amd``
+never imports it.
 
 Every wrapper returns numpy arrays. Neighbour lists come back row-major,
 shape ``(n, k)``; the GPU path stores them transposed and the tests transpose

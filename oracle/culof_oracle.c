@@ -1,9 +1,9 @@
 /*
  * CPU oracle for the LOF forward path.
  *
- * TEST INFRASTRUCTURE ONLY. This file is the checker: it is loaded by the
- * test-suite, by the smoke check and by the benchmark's CPU-baseline leg, and
- * by nothing else. The product path (culof_amd/) never links, loads or calls
+ * TEST INFRASTRUCTURE ONLY. Results of the GPU path are compared with this
+ * file's; it is loaded by the test-suite, by the smoke check and by the
+ * benchmark's CPU-baseline leg, and by nothing else. The product path (culof_amd/) never links, loads or calls
  * it, and has no CPU fallback of any kind.
  *
  * Every function restates one stage of the reference algorithm in plain C,
@@ -106,7 +106,7 @@ void oracle_center(const float* x, int n, int d, int normalize, float* xc, float
  * |x_i|^2 + |x_j|^2 - 2 x_i.x_j (pipeline.cu:182-188, knn.cu:150-151), with the
  * diagonal and a NaN expansion mapped to +inf. In FP32 that expansion cancels:
  * its error is about d u (|x_i|^2 + |x_j|^2), 1e-4 relative to near-neighbour
- * distances on clustered data. This build keys selection by the squared
+ * distances on clustered data. Selection here is keyed by the squared
  * distance in difference form instead,
  *     e(i, j) = sum_t (a_t - b_t)^2  as an FP32 FMA chain in ascending t,
  * whose error is about d u e(i, j), and uses the Gram expansion (over the

@@ -2,8 +2,8 @@
 
 The harness runs culof_amd against scikit-learn with agreement enforced
 (benchmarks/benchmark.py:94-112); the accuracy report checks the figures the
-reference publishes (benchmarks/figures.py:218-267) against the tighter
-targets of this build: median relative difference <= 1e-6, p99 <= 5e-6 and an
+reference publishes (benchmarks/figures.py:218-267) against tighter
+targets: median relative difference <= 1e-6, p99 <= 5e-6 and an
 identical top 1%.
 """
 

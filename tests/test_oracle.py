@@ -80,7 +80,7 @@ def test_fp64_oracle_matches_sklearn(case):
 
 
 @pytest.mark.parametrize("case", GOLDEN_CASES)
-def test_fp32_oracle_within_north_star_tolerance(case):
+def test_fp32_oracle_within_accuracy_targets(case):
     """Median 1e-6, p99 5e-6 against scikit-learn FP64, identical top-1% sets."""
     g = load_golden(case)
     k = int(g["k"])
