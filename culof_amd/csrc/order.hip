@@ -234,7 +234,11 @@ seed_kernel(const float* __restrict__ xs_ord, const int* __restrict__ order, int
     constexpr int kPitch = DP + 4;             // LDS row pitch (floats)
     constexpr int kF4 = kRows * DP / 4;        // float4s per chunk
     constexpr int kLoads = (kF4 + kBlock - 1) / kBlock;
-    __shared__ __attribute__((aligned(16))) float s_q[kQ][DP];
+    // Wide rows: the wave's queries are read by scalar loads (their rows are
+    // wave-uniform), so the LDS pipe serves only the candidate rows; from
+    // LDS every step would read QPW query broadcasts besides one candidate.
+    constexpr bool kScalarQ = DP >= 32;
+    __shared__ __attribute__((aligned(16))) float s_q[kScalarQ ? 1 : kQ][DP];
     __shared__ __attribute__((aligned(16))) float s_c[kRows * kPitch];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -242,12 +246,20 @@ seed_kernel(const float* __restrict__ xs_ord, const int* __restrict__ order, int
     const int w = n < W ? n : W;
     int c0 = p0 + kQ / 2 - w / 2;
     c0 = c0 < 0 ? 0 : (c0 > n - w ? n - w : c0);
-    for (int e = threadIdx.x; e < kQ * DP / 4; e += kBlock) {
-        const int t = e / (DP / 4);
-        const int v = e - t * (DP / 4);
-        const int pos = p0 + t < n ? p0 + t : n - 1;
-        reinterpret_cast<float4*>(&s_q[t][0])[v] =
-            reinterpret_cast<const float4*>(xs_ord + static_cast<size_t>(pos) * STRIDE)[v];
+    if constexpr (!kScalarQ) {
+        for (int e = threadIdx.x; e < kQ * DP / 4; e += kBlock) {
+            const int t = e / (DP / 4);
+            const int v = e - t * (DP / 4);
+            const int pos = p0 + t < n ? p0 + t : n - 1;
+            reinterpret_cast<float4*>(&s_q[t][0])[v] =
+                reinterpret_cast<const float4*>(xs_ord + static_cast<size_t>(pos) * STRIDE)[v];
+        }
+    }
+    const float* qrow[QPW];
+#pragma unroll
+    for (int t = 0; t < QPW; ++t) {
+        const int pos = p0 + wave * QPW + t < n ? p0 + wave * QPW + t : n - 1;
+        qrow[t] = xs_ord + static_cast<size_t>(__builtin_amdgcn_readfirstlane(pos)) * STRIDE;
     }
     float4 nxt[kLoads];
     auto fetch = [&](int ch) {
@@ -290,7 +302,12 @@ seed_kernel(const float* __restrict__ xs_ord, const int* __restrict__ order, int
                 const float4 cv = row[v];
 #pragma unroll
                 for (int t = 0; t < QPW; ++t) {
-                    const float4 qv = reinterpret_cast<const float4*>(&s_q[wave * QPW + t][0])[v];
+                    float4 qv;
+                    if constexpr (kScalarQ) {
+                        qv = reinterpret_cast<const float4*>(qrow[t])[v];
+                    } else {
+                        qv = reinterpret_cast<const float4*>(&s_q[wave * QPW + t][0])[v];
+                    }
                     float df = qv.x - cv.x;
                     e[t] = __builtin_fmaf(df, df, e[t]);
                     df = qv.y - cv.y;
