@@ -39,3 +39,39 @@ def test_accuracy_report_meets_targets(form):
     assert s["p99_rel"] <= 5e-6
     assert s["top1pct_agreement"] == 1.0
     assert 0 <= rep.n_attributed <= rep.n_above
+
+
+@pytest.fixture(scope="module")
+def harness_200k():
+    """The reference's benchmark data at n=200k, d=8 (2,000 wide outliers
+    first), with oracle lists for every outlier row and 200 sampled inliers."""
+    import oracle
+
+    X = bm.blobs_with_outliers(200_000, 8, seed=3)
+    k = 20
+    rows = np.r_[np.arange(2_000), np.random.default_rng(3).choice(np.arange(2_000, 200_000), 200, replace=False)]
+    oi, od, ok = oracle.knn_rows(X, k, 0, 2_000)
+    want = {r: (oi[r], od[r], ok[r]) for r in range(2_000)}
+    for r in rows[2_000:]:
+        i1, d1, k1 = oracle.knn_rows(X, k, int(r), 1)
+        want[int(r)] = (i1[0], d1[0], k1[0])
+    return X, k, rows, want
+
+
+@pytest.mark.parametrize("form", [0, 1], ids=["fp32", "bf16"])
+def test_outlier_rows_bit_exact_on_reference_data(harness_200k, form):
+    """At this size wide outliers whose seeds cover more than 65,536 rows are
+    flagged heavy and take the seeded list pass instead of the collect
+    buffers (1.2 ms of list pass per call on this data); every outlier's list
+    and k-distance, and the sampled inliers', are bit-exact with the oracle in
+    both screen forms."""
+    from _gpu import forward
+
+    X, k, rows, want = harness_200k
+    got = forward(X, k, form=form)
+    for r in rows:
+        wi, wd, wk = want[int(r)]
+        np.testing.assert_array_equal(got["idx"][:, r], wi, err_msg=f"row {r} idx")
+        np.testing.assert_array_equal(got["d2"][:, r].view(np.uint32), wd.view(np.uint32),
+                                      err_msg=f"row {r} d2")
+        assert np.float32(got["kdist2"][r]).view(np.uint32) == np.float32(wk).view(np.uint32), f"row {r} kdist2"
