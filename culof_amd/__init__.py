@@ -11,8 +11,10 @@ transductive fit_predict workflow and is not a full scikit-learn estimator.
     >>> scores = lof(X, k=20)                         # ~1.0 normal, higher = more anomalous
 
 Every call runs the HIP kernels of ``libculof_amd.so`` through the ``_culof``
-extension; there is no CPU fallback. Importing fails if the native library has
-not been built (``python -m culof_amd.build``).
+extension; there is no CPU fallback. The extension loads on first use: any
+call (and ``CudaError``) raises ImportError while the native library is not
+built (``python -m culof_amd.build``), so the builder itself imports from an
+unbuilt tree.
 """
 
 from __future__ import annotations
@@ -31,14 +33,6 @@ try:
 except ImportError:  # pragma: no cover - numpy-only installs
     pass
 
-try:
-    from . import _culof
-except ImportError as exc:  # pragma: no cover - exercised only on unbuilt trees
-    raise ImportError(
-        "culof_amd's native extension is not built; run `python -m culof_amd.build` "
-        f"(original error: {exc})"
-    ) from exc
-
 from ._version import __version__
 
 if TYPE_CHECKING:  # pragma: no cover
@@ -46,9 +40,33 @@ if TYPE_CHECKING:  # pragma: no cover
 
 __all__ = ["LOF", "CudaError", "__version__", "cuda_available", "device_info", "lof"]
 
-cuda_available = _culof.cuda_available
-device_info = _culof.device_info
-CudaError = _culof.CudaError
+_culof_module = None
+
+
+def _ext():
+    """The pybind11 extension ``_culof``, imported on first use."""
+    global _culof_module
+    if _culof_module is None:
+        try:
+            from . import _culof
+        except ImportError as exc:
+            raise ImportError(
+                "culof_amd's native extension is not built; run `python -m culof_amd.build` "
+                f"(original error: {exc})"
+            ) from exc
+        _culof_module = _culof
+    return _culof_module
+
+
+def cuda_available() -> bool:
+    """True when a HIP device is present (reference: include/culof.h:28)."""
+    return bool(_ext().cuda_available())
+
+
+def device_info() -> str:
+    """Name, architecture, CU count and memory of the current device
+    (reference: include/culof.h:31)."""
+    return str(_ext().device_info())
 
 
 def _validate(X: ArrayLike, k: int) -> NDArray[np.float32]:
@@ -126,7 +144,7 @@ def lof(X: ArrayLike, k: int = 20, normalize: bool = False, form: str = "fp32",
     """
     arr = _validate(X, k)
     if form == "fp32" and ties == "index":
-        scores = _culof.lof(arr, k, normalize)
+        scores = _ext().lof(arr, k, normalize)
     else:
         from .autograd import form_code, forward_state
 
@@ -143,7 +161,10 @@ def lof(X: ArrayLike, k: int = 20, normalize: bool = False, form: str = "fp32",
 
 
 def __getattr__(name: str):
-    # The autograd binding needs PyTorch-ROCm; it loads on first use.
+    # The extension's exception type and the autograd binding (which needs
+    # PyTorch-ROCm) load on first use.
+    if name == "CudaError":
+        return _ext().CudaError
     if name in ("lof_torch", "LOFFunction"):
         from . import autograd
 
