@@ -10,9 +10,10 @@
 //   score (density.hip).
 // Scratch is stream-ordered (hipMallocAsync on the call's stream), so calls on
 // different streams or threads never share memory and nothing synchronises
-// the device mid-pipeline. CULOF_TILE_ROWS still splits the query rows into
-// separate launches; results are bit-identical for any tile height because
-// every query row is computed independently.
+// the device mid-pipeline. Query rows go to the kNN kernels in tiles sized to
+// the device memory left free (all rows at once unless memory is short);
+// CULOF_TILE_ROWS forces a tile height. Results are bit-identical for any
+// tile height because every query row is computed independently.
 
 #include "internal.hpp"
 
@@ -93,40 +94,98 @@ namespace {
 // Stream-ordered scratch.
 // ---------------------------------------------------------------------------
 
+// Memory the device's default pool keeps after a call's scratch is freed, so
+// repeated calls do not return it to the driver and fault it back in: up to
+// 1/32 of the device's memory, at most 8 GiB (CULOF_POOL_RETAIN_MB overrides).
+// Larger calls release the excess at their next synchronisation.
 void enable_pool_reuse(int device) {
-    // Keep freed blocks in the device's default pool so repeated calls do not
-    // return memory to the driver and fault it back in.
     static std::mutex mu;
     static std::vector<int> done;
     std::lock_guard<std::mutex> lock(mu);
     if (std::find(done.begin(), done.end(), device) != done.end()) return;
     hipMemPool_t pool;
     if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
-        uint64_t threshold = ~0ull;
+        size_t free_b = 0, total_b = 0;
+        (void)hipMemGetInfo(&free_b, &total_b);
+        uint64_t threshold = std::min<uint64_t>(total_b / 32, 8ull << 30);
+        if (const char* env = std::getenv("CULOF_POOL_RETAIN_MB")) {
+            const long long mb = std::atoll(env);
+            if (mb >= 0) threshold = static_cast<uint64_t>(mb) << 20;
+        }
         (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
     }
     done.push_back(device);
 }
 
+// The reference keeps 128 MiB of device memory free when it sizes its tiles
+// (src/cuda/pipeline.cu:104-108); so does every call here.
+constexpr size_t kFreeMargin = 128ull << 20;
+
+// Device memory a call can still obtain from the stream-ordered allocator:
+// free memory plus what the default pool holds idle, less the margin.
+size_t device_scratch_available() {
+    size_t free_b = 0, total_b = 0;
+    CULOF_CHECK(hipMemGetInfo(&free_b, &total_b));
+    int device = 0;
+    hipMemPool_t pool;
+    uint64_t reserved = 0, used = 0;
+    if (hipGetDevice(&device) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess &&
+        hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &reserved) == hipSuccess &&
+        hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemCurrent, &used) == hipSuccess && reserved > used) {
+        free_b += static_cast<size_t>(reserved - used);
+    }
+    return free_b > kFreeMargin ? free_b - kFreeMargin : 0;
+}
+
+// CULOF_SCRATCH_BUDGET (bytes): an upper bound on one call's scratch, below
+// what the device has free (to leave memory to other work, or to exercise the
+// row tiling on a small budget). 0 when unset.
+size_t scratch_budget_env() {
+    const char* env = std::getenv("CULOF_SCRATCH_BUDGET");
+    if (!env) return 0;
+    const long long v = std::atoll(env);
+    return v > 0 ? static_cast<size_t>(v) : 0;
+}
+
+// A call's device scratch: stream-ordered allocations (hipMallocAsync on the
+// call's stream from the default pool), freed when the call returns.
 class Scratch {
 public:
-    explicit Scratch(hipStream_t s) : stream_(s) {}
+    explicit Scratch(hipStream_t s) : stream_(s), budget_(scratch_budget_env()) {}
     ~Scratch() {
         for (void* p : ptrs_) (void)hipFreeAsync(p, stream_);
     }
     Scratch(const Scratch&) = delete;
     Scratch& operator=(const Scratch&) = delete;
 
+    // Bytes get() reserves for `count` elements: whole 256-byte blocks.
+    template <typename T>
+    static size_t footprint(size_t count) {
+        return (std::max<size_t>(count * sizeof(T), 256) + 255) / 256 * 256;
+    }
+
     template <typename T>
     T* get(size_t count) {
+        const size_t bytes = footprint<T>(count);
         void* p = nullptr;
-        CULOF_CHECK(hipMallocAsync(&p, std::max<size_t>(count * sizeof(T), 256), stream_));
+        CULOF_CHECK(hipMallocAsync(&p, bytes, stream_));
         ptrs_.push_back(p);
+        used_ += bytes;
         return static_cast<T*>(p);
+    }
+
+    // Bytes later get() calls may still take (free device memory and the
+    // pool's idle blocks less the margin, within CULOF_SCRATCH_BUDGET).
+    size_t available() const {
+        size_t avail = device_scratch_available();
+        if (budget_) avail = std::min(avail, budget_ > used_ ? budget_ - used_ : size_t{0});
+        return avail;
     }
 
 private:
     hipStream_t stream_;
+    size_t budget_ = 0;
+    size_t used_ = 0;
     std::vector<void*> ptrs_;
 };
 
@@ -165,14 +224,25 @@ void require_device() {
     if (!device_present()) throw CudaError("no HIP device available");
 }
 
-// Rows per sweep launch: CULOF_TILE_ROWS when set (the reference's knob,
-// src/cuda/pipeline.cu:99-103), else every row in one launch.
-int tile_rows_for(int n_points) {
+// Query rows per launch (the reference's tile_rows_for,
+// src/cuda/pipeline.cu:99-110): CULOF_TILE_ROWS when set, else the largest of
+// rows, rows / 2, rows / 4, ... (down to kMinTileRows) whose scratch,
+// bytes_for(tile), fits in `avail`. The reference sizes a rows x n Gram tile;
+// here nothing is n wide, and what grows with the tile is the collect
+// buffers (about 8-64 KB per query row), so all rows usually go in one launch.
+// Every query row is computed on its own, so any tile height gives the same
+// bits; a run split into tiles takes the non-symmetric collect pass.
+constexpr int kMinTileRows = 256;
+
+template <class Bytes>
+int tile_rows_for(int rows, size_t avail, Bytes&& bytes_for) {
     if (const char* env = std::getenv("CULOF_TILE_ROWS")) {
         const int forced = std::atoi(env);
-        if (forced > 0) return std::min(forced, n_points);
+        if (forced > 0) return std::min(forced, rows);
     }
-    return n_points;
+    int tile = rows;
+    while (tile > kMinTileRows && bytes_for(tile) > avail) tile = (tile + 1) / 2;
+    return std::max(tile, std::min(rows, 1));
 }
 
 // Device-resident inputs of the selection stage, built by prepare().
@@ -194,11 +264,15 @@ Prepared prepare(const float* points, int n, int d, bool normalize, int form, Sc
     float* norms = scratch.get<float>(n);
     double* work = scratch.get<double>(static_cast<size_t>(kPrepWork) * d);
     int* guard = scratch.get<int>(1);
-    float* xs = normalize ? nullptr : scratch.get<float>(cells);
-    CULOF_CHECK(hipMemsetAsync(guard, 0, sizeof(int), s));
+    // Key rows: the input itself when it already has the padded stride and
+    // whole-float4 alignment, else a padded copy (the z-scored rows when
+    // normalising).
+    const bool in_place = !normalize && p.stride == d && reinterpret_cast<uintptr_t>(points) % 16 == 0;
+    float* xs = (normalize || in_place) ? nullptr : scratch.get<float>(cells);
+    zero_fill(guard, sizeof(int), s);
     // Centre always; scale only when asked (src/cuda/pipeline.cu:162-165).
     center_and_norms(points, n, d, normalize, xc, p.stride, norms, work, xs, guard, s);
-    p.xs = normalize ? xc : xs;
+    p.xs = normalize ? xc : (in_place ? points : xs);
     p.xc = xc;
     p.norms = norms;
     p.guard = guard;
@@ -268,10 +342,23 @@ void knn_rows(const Prepared& p, int n, int d, int k, int form, int row0, int ro
                        tie_rows, tie_count, band.rows ? &band : nullptr, s);
         return;
     }
-    const int tile = tile_rows_for(rows);
     Path path = Path::kRadix;
     if (method == 0) path = choose_path(d, k, form, p.stride);
     if (method == 2) path = Path::kValu;
+    const bool order = path == Path::kMfma && ordering_enabled() && order_supported(p.stride);
+    // Only the matrix-core path keeps per-row scratch (its collect buffers).
+    auto tile_bytes = [&](int t) {
+        if (path != Path::kMfma) return size_t{0};
+        size_t b = Scratch::footprint<unsigned char>(mfma_scratch_bytes(n, p.stride, form, t, k, order));
+        if (order) {
+            b += Scratch::footprint<unsigned char>(order_scratch_bytes(n, p.stride)) +
+                 Scratch::footprint<float>(static_cast<size_t>(n));
+        }
+        return b;
+    };
+    const size_t fixed = 2 * Scratch::footprint<int>(static_cast<size_t>(rows));
+    const size_t avail = scratch.available();
+    const int tile = tile_rows_for(rows, avail > fixed ? avail - fixed : 0, tile_bytes);
     if (path == Path::kRadix) {
         for (int r0 = 0; r0 < rows; r0 += tile) {
             radix_knn(p.xs, p.stride, n, d, k, row0 + r0, std::min(tile, rows - r0), nullptr,
@@ -281,13 +368,12 @@ void knn_rows(const Prepared& p, int n, int d, int k, int form, int row0, int ro
     }
     int* fb_count = scratch.get<int>(1);
     int* fb_rows = scratch.get<int>(static_cast<size_t>(rows));
-    CULOF_CHECK(hipMemsetAsync(fb_count, 0, sizeof(int), s));
+    zero_fill(fb_count, sizeof(int), s);
     void* mfma_scratch = nullptr;
     CandidateOrder ord;
     const int* qorder = nullptr;
     float* tseed = nullptr;
     if (path == Path::kMfma) {
-        const bool order = ordering_enabled() && order_supported(p.stride);
         mfma_scratch = scratch.get<unsigned char>(mfma_scratch_bytes(n, p.stride, form, tile, k, order));
         if (order) {
             // Candidates in locality order, lists seeded from nearby candidates;
@@ -304,13 +390,13 @@ void knn_rows(const Prepared& p, int n, int d, int k, int form, int row0, int ro
         if (path == Path::kMfma) {
             mfma_knn(p.xs, p.xc, p.stride, p.norms, p.guard, n, d, k, form, row0 + r0, r,
                      tseed ? &ord : nullptr, qorder ? qorder + r0 : nullptr, tseed, mfma_scratch,
-                     nbr_idx, nbr_d2, kdist2, fb_rows, fb_count, s);
+                     nbr_idx, nbr_d2, kdist2, fb_rows, fb_count, rows, s);
         } else {
             sweep_knn(p.xs, p.xc, p.stride, p.norms, p.guard, n, d, k, row0 + r0, r, nbr_idx,
-                      nbr_d2, kdist2, fb_rows, fb_count, s);
+                      nbr_d2, kdist2, fb_rows, fb_count, rows, s);
         }
     }
-    radix_knn(p.xs, p.stride, n, d, k, 0, 0, fb_rows, fb_count, nbr_idx, nbr_d2, kdist2, s);
+    radix_knn(p.xs, p.stride, n, d, k, 0, rows, fb_rows, fb_count, nbr_idx, nbr_d2, kdist2, s);
 }
 
 void forward(const float* points, int n, int d, int k, bool normalize, int form, float* scores,
@@ -374,9 +460,9 @@ void novelty(const float* points, int n, int d, const float* queries, int m, int
         // (huge norms, fewer than k finite keys) take the exact radix select.
         int* fb_count = scratch.get<int>(1);
         int* fb_rows = scratch.get<int>(static_cast<size_t>(m));
-        CULOF_CHECK(hipMemsetAsync(fb_count, 0, sizeof(int), s));
+        zero_fill(fb_count, sizeof(int), s);
         sweep_query_knn(qkeys, qc, qnorms, m, p.xs, p.xc, p.stride, p.norms, p.guard, n, d, k, qidx, qd2,
-                        qkd, fb_rows, fb_count, s);
+                        qkd, fb_rows, fb_count, m, s);
         radix_query_knn(qkeys, p.xs, p.stride, n, d, k, m, fb_rows, fb_count, qidx, qd2, qkd, s);
     } else {
         radix_query_knn(qkeys, p.xs, p.stride, n, d, k, m, nullptr, nullptr, qidx, qd2, qkd, s);

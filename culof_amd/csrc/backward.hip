@@ -428,8 +428,8 @@ void lof_backward(const float* keyrows, int n, int d, int k, const int* nbr_idx,
     ProfileScope prof("lof_backward", s);
     point_m_kernel<<<grid_for(nn), kBlock, 0, s>>>(nbr_idx, nbr_d2, kdist2, n, k, m, lrd);
     point_s_kernel<<<grid_for(nn), kBlock, 0, s>>>(nbr_idx, lrd, m, grad_scores, n, k, S, a);
-    CULOF_CHECK(hipMemsetAsync(count, 0, nn * sizeof(int), s));
-    CULOF_CHECK(hipMemsetAsync(cursor, 0, nn * sizeof(int), s));
+    zero_fill(count, nn * sizeof(int), s);
+    zero_fill(cursor, nn * sizeof(int), s);
     rev_count_kernel<<<grid_for(nk), kBlock, 0, s>>>(nbr_idx, nk, count);
     scan_blocks_kernel<<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(count, n, offset, totals);
     scan_totals_kernel<<<1, kBlock, 0, s>>>(totals, static_cast<int>(blocks));

@@ -27,6 +27,19 @@ constexpr uint32_t kInfBits = 0x7F800000u;
 constexpr int kPrepWork = 256 + 3;  // doubles per feature of center_and_norms' work buffer
 constexpr int kMaxBackwardDims = 256;
 
+// Appends v to a device queue of `cap` entries. Every producer appends a row
+// at most once per queue, and queues are sized for every row, so the cap is
+// never reached; an append past it is dropped rather than written out of
+// bounds, and consumers read min(*count, cap) entries.
+__device__ __forceinline__ void queue_push(int* list, int* count, int cap, int v) {
+    const int at = atomicAdd(count, 1);
+    if (at < cap) list[at] = v;
+}
+
+// Zeroes `bytes` (a multiple of 4) of device memory on `stream` with a kernel:
+// unlike hipMemsetAsync it records a plain kernel node under graph capture.
+void zero_fill(void* dst, size_t bytes, hipStream_t stream);
+
 // Precision forms of the distance screen (culof_form in culof_c.h).
 constexpr int kFormPlain = 0;  // FP32 screen
 constexpr int kFormBF16 = 1;   // BF16 MFMA screen
@@ -82,18 +95,20 @@ bool sweep_supported(int n_dims, int k);
 // Queries [row0, row0 + rows). `keyrows` and `centred` share the padded
 // stride; `norms` are of the centred rows; *guard set means every row goes to
 // the fallback list. Rows whose k-th key is +inf are appended to
-// fallback_rows with fallback_count incremented; the radix path redoes them.
+// fallback_rows (fallback_cap entries) with fallback_count incremented; the
+// radix path redoes them.
 void sweep_knn(const float* keyrows, const float* centred, int stride, const float* norms,
                const int* guard, int n_points, int n_dims, int k, int row0, int rows,
                int* nbr_idx, float* nbr_d2, float* kdist2, int* fallback_rows,
-               int* fallback_count, hipStream_t stream);
+               int* fallback_count, int fallback_cap, hipStream_t stream);
 // The same sweep for rows of another set (qkeys / qcentred / qnorms, same
 // stride and transform) against all n_points rows, nothing excluded; outputs
 // with stride n_queries; rows the sweep cannot settle go to fallback_rows.
 void sweep_query_knn(const float* qkeys, const float* qcentred, const float* qnorms, int n_queries,
                      const float* keyrows, const float* centred, int stride, const float* norms,
                      const int* guard, int n_points, int n_dims, int k, int* nbr_idx, float* nbr_d2,
-                     float* kdist2, int* fallback_rows, int* fallback_count, hipStream_t stream);
+                     float* kdist2, int* fallback_rows, int* fallback_count, int fallback_cap,
+                     hipStream_t stream);
 
 // ---- mfma.hip: matrix-core screened sweep (FP32 or BF16 screen) ----
 // Row stride the form's screen expects (BF16 needs multiples of 16).
@@ -107,24 +122,27 @@ struct CandidateOrder;
 // ord (optional): stream candidates in that order; qlist (optional): the
 // original indices of the `rows` queries (else row0 + local); tseed
 // (optional, indexed by original query index): an upper bound of each
-// query's k-th key from seed_thresholds.
+// query's k-th key from seed_thresholds. fallback_rows holds fallback_cap
+// entries (every row of the call).
 void mfma_knn(const float* keyrows, const float* centred, int stride, const float* norms,
               const int* guard, int n_points, int n_dims, int k, int form, int row0, int rows,
               const CandidateOrder* ord, const int* qlist, const float* tseed,
               void* scratch, int* nbr_idx, float* nbr_d2, float* kdist2, int* fallback_rows,
-              int* fallback_count, hipStream_t stream);
+              int* fallback_count, int fallback_cap, hipStream_t stream);
 
 // ---- radix.hip: per-row exact radix select (reference src/cuda/knn.cu) ----
 // Recompute mode: keys rebuilt from `keyrows` (row stride `stride`) every
 // pass. With `row_count` (device) null the query rows are [row0, row0 + rows);
-// otherwise they are row_list[0 .. *row_count) and the launch is a fixed-size
-// grid-stride loop, so no host read-back is needed.
+// otherwise they are row_list[0 .. min(*row_count, rows)) (`rows` is then the
+// list's capacity) and the launch is a fixed-size grid-stride loop, so no host
+// read-back is needed.
 void radix_knn(const float* keyrows, int stride, int n_points, int n_dims, int k, int row0,
                int rows, const int* row_list, const int* row_count, int* nbr_idx, float* nbr_d2,
                float* kdist2, hipStream_t stream);
 // Query mode: rows of `qrows` (another row set, same stride) against all
 // n_points key rows, nothing excluded; outputs with stride n_queries. All
-// queries, or row_list[0 .. *row_count) (device) when row_count is given.
+// queries, or row_list[0 .. min(*row_count, n_queries)) (device) when
+// row_count is given.
 void radix_query_knn(const float* qrows, const float* keyrows, int stride, int n_points, int n_dims,
                      int k, int n_queries, const int* row_list, const int* row_count, int* nbr_idx,
                      float* nbr_d2, float* kdist2, hipStream_t stream);
@@ -169,10 +187,11 @@ struct TieBand {
 int tie_band_capacity(int n_points, int n_dims, int stride, int form, int rows);
 size_t tie_band_scratch_bytes(int n_points, int n_dims, int form, int cap);
 // Collect pass over the queued rows with their bounds as thresholds, then the
-// FP64 band select; rows it cannot hold are appended to brute_rows.
+// FP64 band select; rows it cannot hold are appended to brute_rows (brute_cap
+// entries).
 void tie_band_select(const float* keyrows, int stride, int n_points, int k, const TieBand& band,
                      int* nbr_idx, float* nbr_d2, float* kdist2, int* brute_rows, int* brute_count,
-                     hipStream_t stream);
+                     int brute_cap, hipStream_t stream);
 
 // idx1 / d21: the (k+1)-lists of rows [row0, row0 + rows) (stride n). Writes
 // the k-lists of those rows; tie_rows needs `rows` ints, tie_count one int.

@@ -372,6 +372,7 @@ struct MfmaArgs {
     float* kdist2;
     int* fb_rows;
     int* fb_count;
+    int fb_cap;          // entries fb_rows holds
     const int* heavy;    // symmetric pass: 1 for rows (by position) queued for the list pass up front
 };
 
@@ -637,7 +638,7 @@ mfma_sweep_kernel(MfmaArgs p) {
     const int n = p.n;
 
     if (*p.guard) {
-        if (valid && h == 0 && blockIdx.y == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+        if (valid && h == 0 && blockIdx.y == 0) queue_push(p.fb_rows, p.fb_count, p.fb_cap, qi);
         return;
     }
 
@@ -785,7 +786,7 @@ mfma_sweep_kernel(MfmaArgs p) {
                         if (t < p.k) stage[(p.k - 1 - t) * 64 + lane] = a[t];
                     }
                     if (!emit_list(stage + lane, p.k, n, qi, p.nbr_idx, p.nbr_d2, p.kdist2)) {
-                        p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+                        queue_push(p.fb_rows, p.fb_count, p.fb_cap, qi);
                     }
                 }
             }
@@ -821,7 +822,7 @@ mfma_collect_kernel(MfmaArgs p) {
     const int qi = p.qlist ? p.qlist[valid ? local : 0] : p.row0 + (valid ? local : 0);
 
     if (*p.guard) {
-        if (valid && h == 0 && blockIdx.y == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+        if (valid && h == 0 && blockIdx.y == 0) queue_push(p.fb_rows, p.fb_count, p.fb_cap, qi);
         return;
     }
 
@@ -938,7 +939,7 @@ mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2
     if (*p.guard) {
 #pragma unroll
         for (int t = 0; t < QT; ++t)
-            if (valid[t] && h == 0 && blockIdx.y == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi[t];
+            if (valid[t] && h == 0 && blockIdx.y == 0) queue_push(p.fb_rows, p.fb_count, p.fb_cap, qi[t]);
         return;
     }
 
@@ -1102,11 +1103,18 @@ constexpr int kHeavyChunk = 64;  // sample rows staged in LDS at a time
 template <int DP>
 __global__ void __launch_bounds__(256)
 heavy_rows_kernel(const float* __restrict__ xs_ord, const int* __restrict__ order,
-                  const float* __restrict__ tseed, int n, int m, int* __restrict__ heavy,
-                  int* __restrict__ list_rows, int* __restrict__ list_count, int list_cap) {
+                  const float* __restrict__ tseed, const int* __restrict__ guard, int n, int m,
+                  int* __restrict__ heavy, int* __restrict__ list_rows, int* __restrict__ list_count,
+                  int list_cap) {
     __shared__ __attribute__((aligned(16))) float s_smp[kHeavyChunk * DP];
     const int q = blockIdx.x * 256 + threadIdx.x;
     const bool valid = q < n;
+    if (*guard) {
+        // The symmetric pass hands every row to the radix path: queue none
+        // here, or those rows would reach the fallback queue twice.
+        if (valid) heavy[q] = 0;
+        return;
+    }
     float xq[DP];
     const float* row = xs_ord + static_cast<size_t>(valid ? q : 0) * DP;
 #pragma unroll
@@ -1304,7 +1312,7 @@ __device__ void select_words(const MfmaArgs& p, const uint64_t* s_words, int tot
     }
     const uint64_t ans = (static_cast<uint64_t>(kh) << 32) | lo;
     if (kh >= kInfBits) {
-        if (lane == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+        if (lane == 0) queue_push(p.fb_rows, p.fb_count, p.fb_cap, qi);
         return;
     }
     int off = 0;
@@ -1367,17 +1375,17 @@ collect_select_kernel(MfmaArgs p) {
             } else if (at2 < p.list_cap) {
                 p.list_rows[at2] = qi;
             } else {
-                p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+                queue_push(p.fb_rows, p.fb_count, p.fb_cap, qi);
             }
         }
         return;
     }
     if (total > kWaveMax) {
-        if (lane == 0) p.big_rows[atomicAdd(p.big_count, 1)] = local;
+        if (lane == 0) queue_push(p.big_rows, p.big_count, p.rows, local);
         return;
     }
     if (total < p.k) {  // cannot happen with a valid seed; stay exact regardless
-        if (lane == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+        if (lane == 0) queue_push(p.fb_rows, p.fb_count, p.fb_cap, qi);
         return;
     }
     // Exact keys straight from the slice buffers, every lane busy: entry e's
@@ -1460,7 +1468,7 @@ big_select_kernel(MfmaArgs p, const int* __restrict__ rows_list, const int* __re
                 if (at < p.ov_cap) {
                     p.ov_rows[at] = qi;
                 } else {
-                    p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+                    queue_push(p.fb_rows, p.fb_count, p.fb_cap, qi);
                 }
             }
             continue;
@@ -1524,7 +1532,7 @@ big_select_kernel(MfmaArgs p, const int* __restrict__ rows_list, const int* __re
         }
         if (tid < 64) {
             if (static_cast<uint32_t>(s_w[p.k - 1] >> 32) >= kInfBits) {
-                if (tid == 0) p.fb_rows[atomicAdd(p.fb_count, 1)] = qi;
+                if (tid == 0) queue_push(p.fb_rows, p.fb_count, p.fb_cap, qi);
             } else {
                 emit_selected(p, s_w, qi, tid);
             }
@@ -1585,7 +1593,7 @@ band_select_kernel(MfmaArgs p, const double* __restrict__ bound_of, int* __restr
         }
         __syncthreads();
         if (s_pre[0] < 0) {
-            if (threadIdx.x == 0) brute_rows[atomicAdd(brute_count, 1)] = qi;
+            if (threadIdx.x == 0) queue_push(brute_rows, brute_count, p.fb_cap, qi);
             continue;
         }
         const int total = s_pre[nb];
@@ -1611,7 +1619,7 @@ band_select_kernel(MfmaArgs p, const double* __restrict__ bound_of, int* __restr
         __syncthreads();
         const int m = s_m;
         if (m < p.k) {  // cannot happen (the FP32 k-list lies in the band); stay exact regardless
-            if (threadIdx.x == 0) brute_rows[atomicAdd(brute_count, 1)] = qi;
+            if (threadIdx.x == 0) queue_push(brute_rows, brute_count, p.fb_cap, qi);
             continue;
         }
         // Rank of each entry in (e64, index) order; the k-th is the one of rank k - 1.
@@ -1653,7 +1661,7 @@ __global__ void __launch_bounds__(64)
 merge_slices_kernel(const uint64_t* __restrict__ partial, int slices, int rows, int row0,
                     const int* __restrict__ qlist, const int* __restrict__ row_count, int n, int k, int* __restrict__ nbr_idx,
                     float* __restrict__ nbr_d2, float* __restrict__ kdist2,
-                    int* __restrict__ fb_rows, int* __restrict__ fb_count) {
+                    int* __restrict__ fb_rows, int* __restrict__ fb_count, int fb_cap) {
     __shared__ uint64_t stage[KMAX][64];
     const int lane = threadIdx.x;
     const int local = blockIdx.x * 64 + lane;
@@ -1674,7 +1682,7 @@ merge_slices_kernel(const uint64_t* __restrict__ partial, int slices, int rows, 
         if (t < k) stage[k - 1 - t][lane] = a[t];
     }
     if (!emit_list(&stage[0][lane], k, n, qi, nbr_idx, nbr_d2, kdist2)) {
-        fb_rows[atomicAdd(fb_count, 1)] = qi;
+        queue_push(fb_rows, fb_count, fb_cap, qi);
     }
 }
 
@@ -1743,7 +1751,7 @@ void launch_mfma(const MfmaArgs& a, hipStream_t stream) {
         ProfileScope prof("merge_slices", stream);
         merge_slices_kernel<KMAX><<<(a.rows + 63) / 64, 64, 0, stream>>>(
             a.partial, a.slices, a.rows, a.row0, a.qlist, a.row_count, a.n, a.k, a.nbr_idx, a.nbr_d2, a.kdist2,
-            a.fb_rows, a.fb_count);
+            a.fb_rows, a.fb_count, a.fb_cap);
     }
 }
 
@@ -1985,7 +1993,9 @@ size_t mfma_scratch_bytes(int n_points, int dp, int form, int rows, int k, bool 
                            padded * sizeof(float) + padded / 32 * sizeof(float2) +
                            static_cast<size_t>(sym_chunks(n_points)) * (kChunkWords * sizeof(uint64_t) + 2 * sizeof(int)) +
                            2048;
-        const size_t first = std::max(slots * (kSelectMax + 1) * sizeof(uint32_t), sym) + 256;
+        // (the symmetric pass serves full runs only)
+        const bool sym_run = rows == n_points && n_points >= kSymMinRows;
+        const size_t first = std::max(slots * (kSelectMax + 1) * sizeof(uint32_t), sym_run ? sym : 0) + 256;
         lists = first + static_cast<size_t>(rows) * 2 * sizeof(int) + 4 * sizeof(int) +
                 second * kSecondCap * sizeof(uint32_t) + second * sizeof(int) +
                 static_cast<size_t>(std::min(rows, kListRows)) * (sizeof(int) +
@@ -2000,7 +2010,7 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
               const int* guard, int n_points, int n_dims, int k, int form, int row0, int rows,
               const CandidateOrder* ord, const int* qlist, const float* tseed,
               void* scratch, int* nbr_idx, float* nbr_d2, float* kdist2, int* fallback_rows,
-              int* fallback_count, hipStream_t stream) {
+              int* fallback_count, int fallback_cap, hipStream_t stream) {
     const int dp = mfma_padded_dims(n_dims, form);
     if (stride != dp) throw std::invalid_argument("mfma_knn: row stride mismatch");
     if (rows <= 0) return;
@@ -2024,7 +2034,7 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
                   qlist, tseed, madj, norms, guard, n_points, k, c, row0, rows,
                   slices, nullptr, collect ? 1 : 0, nullptr, nullptr, nullptr, kSelectMax,
                   nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr, 0, nbr_idx, nbr_d2,
-                  kdist2, fallback_rows, fallback_count};
+                  kdist2, fallback_rows, fallback_count, fallback_cap};
     MfmaArgs second = args;
     MfmaArgs list = args;
     // A full run in candidate order computes each pair of tiles once.
@@ -2053,9 +2063,9 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
             sb.chunk_count = static_cast<int*>(take(chunks * sizeof(int)));
             sb.next_chunk = static_cast<int*>(take(sizeof(int)));
             sb.col_over = static_cast<int*>(take(static_cast<size_t>(rows) * sizeof(int)));
-            CULOF_CHECK(hipMemsetAsync(args.ccount, 0, slots * sizeof(int), stream));
-            CULOF_CHECK(hipMemsetAsync(sb.next_chunk, 0, sizeof(int), stream));
-            CULOF_CHECK(hipMemsetAsync(sb.col_over, 0, static_cast<size_t>(rows) * sizeof(int), stream));
+            zero_fill(args.ccount, slots * sizeof(int), stream);
+            zero_fill(sb.next_chunk, sizeof(int), stream);
+            zero_fill(sb.col_over, static_cast<size_t>(rows) * sizeof(int), stream);
         } else {
             const size_t slots = static_cast<size_t>(rows) * slices;
             args.cbuf = static_cast<uint32_t*>(take(slots * kSelectMax * sizeof(uint32_t)));
@@ -2067,7 +2077,7 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
         int* counters = static_cast<int*>(take(4 * sizeof(int)));
         args.big_count = counters;
         args.ov_count = counters + 1;
-        CULOF_CHECK(hipMemsetAsync(counters, 0, 4 * sizeof(int), stream));
+        zero_fill(counters, 4 * sizeof(int), stream);
         // The second pass: the overflowed rows, with large buffers.
         second = args;
         second.qlist = args.ov_rows;
@@ -2123,8 +2133,8 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
             const int m = std::min(n_points, std::clamp(8192 / dp, 64, 512));
             auto go = [&](auto D) {
                 heavy_rows_kernel<decltype(D)::value><<<(n_points + 255) / 256, 256, 0, stream>>>(
-                    ord->xs_ord, ord->order, tseed, n_points, m, heavy, second.ov_rows, second.ov_count,
-                    second.ov_cap);
+                    ord->xs_ord, ord->order, tseed, guard, n_points, m, heavy, second.ov_rows,
+                    second.ov_count, second.ov_cap);
                 return true;
             };
             for_dp<ScreenF32>(dp, go);
@@ -2179,7 +2189,7 @@ size_t tie_band_scratch_bytes(int n_points, int n_dims, int form, int cap) {
 
 void tie_band_select(const float* keyrows, int stride, int n_points, int k, const TieBand& band,
                      int* nbr_idx, float* nbr_d2, float* kdist2, int* brute_rows, int* brute_count,
-                     hipStream_t stream) {
+                     int brute_cap, hipStream_t stream) {
     const int dp = stride;
     const bool bf16 = band.form == kFormBF16;
     const float c = bf16 ? ScreenBF16::slack(dp) : ScreenF32::slack(dp);
@@ -2200,7 +2210,7 @@ void tie_band_select(const float* keyrows, int stride, int n_points, int k, cons
     MfmaArgs a{xa, band.centred, keyrows, keyrows, nullptr, band.rows, band.tband, madj, band.norms,
                band.guard, n_points, k, c, 0, band.cap, slices, nullptr, 1, band.count, cbuf, ccount,
                kBandCap, nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr, 0, nbr_idx, nbr_d2,
-               kdist2, brute_rows, brute_count};
+               kdist2, brute_rows, brute_count, brute_cap};
     auto go_bf16 = [&](auto D) {
         launch_band<ScreenBF16, decltype(D)::value>(a, band.bound, brute_rows, brute_count, stream);
         return true;

@@ -360,8 +360,8 @@ void launch_order(const float* xs, int n, int* cell1, int* hist1, int* offset1, 
                   int* order1, float* piv2, int* key, int* hist, int* offset, int* cursor,
                   int* order, int* inv, hipStream_t s) {
     const unsigned grid = static_cast<unsigned>((n + kBlock - 1) / kBlock);
-    CULOF_CHECK(hipMemsetAsync(hist1, 0, sizeof(int) * kPiv, s));
-    CULOF_CHECK(hipMemsetAsync(hist, 0, sizeof(int) * kKeys, s));
+    zero_fill(hist1, sizeof(int) * kPiv, s);
+    zero_fill(hist, sizeof(int) * kKeys, s);
     level1_kernel<DP><<<grid, kBlock, 0, s>>>(xs, n, cell1, hist1);
     scan_bins_kernel<<<1, kBlock, 0, s>>>(hist1, kPiv, offset1, cursor1);
     scatter_few_kernel<<<grid, kBlock, 0, s>>>(cell1, n, offset1, cursor1, order1);

@@ -14,6 +14,9 @@
 
 #include "internal.hpp"
 
+#include <algorithm>
+#include <stdexcept>
+
 namespace culof {
 namespace detail {
 
@@ -119,7 +122,31 @@ center_apply_kernel(const float* __restrict__ x, int n, int d, const double* __r
     if (guard && !(nf <= 2.1267647e37f)) atomicOr(guard, 1);  // FLT_MAX / 16, NaN
 }
 
+// Zero fill, 16 bytes per store where the region allows.
+__global__ void __launch_bounds__(256)
+zero_words_kernel(uint32_t* __restrict__ dst, size_t words) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if ((reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
+        uint4* d4 = reinterpret_cast<uint4*>(dst);
+        const size_t quads = words / 4;
+        for (size_t q = i; q < quads; q += stride) d4[q] = make_uint4(0u, 0u, 0u, 0u);
+        for (size_t w = 4 * quads + i; w < words; w += stride) dst[w] = 0u;
+        return;
+    }
+    for (; i < words; i += stride) dst[i] = 0u;
+}
+
 }  // namespace
+
+void zero_fill(void* dst, size_t bytes, hipStream_t stream) {
+    if (bytes == 0) return;
+    if (bytes % 4 != 0) throw std::invalid_argument("zero_fill: size must be whole 32-bit words");
+    const size_t words = bytes / 4;
+    const unsigned grid = static_cast<unsigned>(std::min<size_t>((words / 4 + 255) / 256 + 1, 2048));
+    zero_words_kernel<<<grid, 256, 0, stream>>>(static_cast<uint32_t*>(dst), words);
+    CULOF_CHECK(hipGetLastError());
+}
 
 void center_and_norms(const float* points, int n_points, int n_dims, bool scale, float* centred,
                       int stride, float* norms, double* work, float* copy, int* guard,

@@ -20,6 +20,8 @@
 
 #include "internal.hpp"
 
+#include <algorithm>
+
 namespace culof {
 namespace detail {
 
@@ -205,7 +207,7 @@ radix_recompute_kernel(const float* __restrict__ xs, int stride, int d, int n, i
                        float* __restrict__ kdist2) {
     extern __shared__ float s_q[];  // stride floats
     __shared__ RadixShared sh;
-    const int total = count ? *count : rows;
+    const int total = count ? min(*count, rows) : rows;  // rows: the list's capacity
     for (int r = blockIdx.x; r < total; r += gridDim.x) {
         const int i = row_list ? row_list[r] : row0 + r;
         __syncthreads();  // s_q reuse across rows
@@ -227,7 +229,7 @@ radix_query_kernel(const float* __restrict__ qrows, const float* __restrict__ xs
                    float* __restrict__ kdist2) {
     extern __shared__ float s_q[];  // stride floats
     __shared__ RadixShared sh;
-    const int total = count ? *count : n_queries;
+    const int total = count ? min(*count, n_queries) : n_queries;
     for (int i = blockIdx.x; i < total; i += gridDim.x) {
         const int r = row_list ? row_list[i] : i;
         __syncthreads();  // s_q reuse across rows
@@ -256,7 +258,7 @@ void radix_knn(const float* keyrows, int stride, int n_points, int n_dims, int k
                int rows, const int* row_list, const int* row_count, int* nbr_idx, float* nbr_d2,
                float* kdist2, hipStream_t stream) {
     ProfileScope prof("radix_knn", stream);
-    const int grid = row_count ? 2048 : rows;
+    const int grid = row_count ? std::min(rows, 2048) : rows;
     if (grid <= 0) return;
     radix_recompute_kernel<<<grid, kBlock, sizeof(float) * stride, stream>>>(
         keyrows, stride, n_dims, n_points, k, row0, rows, row_list, row_count, nbr_idx, nbr_d2,

@@ -116,7 +116,7 @@ sweep_kernel(const float* __restrict__ xs, const float* __restrict__ xc,
              const float* __restrict__ norms, const int* __restrict__ guard, int n, int k,
              float screen_c, int row0, int rows, int* __restrict__ nbr_idx,
              float* __restrict__ nbr_d2, float* __restrict__ kdist2, int* __restrict__ fb_rows,
-             int* __restrict__ fb_count, const float* __restrict__ qxs, const float* __restrict__ qxc,
+             int* __restrict__ fb_count, int fb_cap, const float* __restrict__ qxs, const float* __restrict__ qxc,
              const float* __restrict__ qnorms, int same_set, int out_stride) {
     using Shape = SweepShape<DP>;
     constexpr int CH = Shape::kChunk;
@@ -148,7 +148,7 @@ sweep_kernel(const float* __restrict__ xs, const float* __restrict__ xc,
     if (*guard) {
         // Some norm is too large for the screen to stay finite: the exact
         // radix path takes every row.
-        if (wave == 0 && valid) fb_rows[atomicAdd(fb_count, 1)] = qi;
+        if (wave == 0 && valid) queue_push(fb_rows, fb_count, fb_cap, qi);
         return;
     }
 
@@ -190,7 +190,11 @@ sweep_kernel(const float* __restrict__ xs, const float* __restrict__ xc,
     __syncthreads();  // every bound initialised before any wave reads one
 
     const int n_chunks = (n + CH - 1) / CH;
-    const int c0 = (row0 + blockIdx.x * 64) / CH;  // rotate: start at own rows
+    // Rotate the start so neighbouring blocks read different chunks first (a
+    // block of the candidate set starts at its own rows). Query rows of
+    // another set may outnumber the candidates: the start is taken modulo the
+    // chunk count so every block still visits each chunk exactly once.
+    const int c0 = ((row0 + static_cast<int>(blockIdx.x) * 64) / CH) % n_chunks;
 
     // Register prefetch of a chunk's rows and adjusted norms.
     float4 pre_c[NV];
@@ -357,7 +361,7 @@ sweep_kernel(const float* __restrict__ xs, const float* __restrict__ xc,
         // Fewer than k finite keys: NaN keys (read as +inf, like the
         // reference's) may then belong to the set, and the screen never admits
         // them. The exact radix path redoes the row.
-        fb_rows[atomicAdd(fb_count, 1)] = qi;
+        queue_push(fb_rows, fb_count, fb_cap, qi);
         return;
     }
     kdist2[qi] = __uint_as_float(kth);
@@ -395,6 +399,7 @@ struct SweepArgs {
     float* kdist2;
     int* fb_rows;
     int* fb_count;
+    int fb_cap;  // entries fb_rows holds
     // Query rows: the candidates themselves (same_set, the point itself
     // excluded, outputs at stride n) or another row set (outputs at stride rows).
     const float* qxs;
@@ -409,7 +414,7 @@ void launch_sweep(const SweepArgs& a, hipStream_t stream) {
     const int grid = (a.rows + 63) / 64;
     sweep_kernel<DP, KMAX><<<grid, 64 * kWaves, 0, stream>>>(
         a.xs, a.xc, a.norms, a.guard, a.n, a.k, a.screen_c, a.row0, a.rows, a.nbr_idx, a.nbr_d2,
-        a.kdist2, a.fb_rows, a.fb_count, a.qxs, a.qxc, a.qnorms, a.same_set, a.out_stride);
+        a.kdist2, a.fb_rows, a.fb_count, a.fb_cap, a.qxs, a.qxc, a.qnorms, a.same_set, a.out_stride);
 }
 
 template <int DP>
@@ -462,14 +467,14 @@ bool sweep_supported(int n_dims, int k) { return dispatch(padded_dims(n_dims), k
 void sweep_knn(const float* keyrows, const float* centred, int stride, const float* norms,
                const int* guard, int n_points, int n_dims, int k, int row0, int rows,
                int* nbr_idx, float* nbr_d2, float* kdist2, int* fallback_rows,
-               int* fallback_count, hipStream_t stream) {
+               int* fallback_count, int fallback_cap, hipStream_t stream) {
     if (stride != padded_dims(n_dims)) throw std::invalid_argument("sweep_knn: row stride mismatch");
     if (rows <= 0) return;
     ProfileScope prof("sweep_knn", stream);
     const SweepArgs args{keyrows, centred, norms, guard, n_points, k,
                          static_cast<float>(n_dims + 16) * 5.9604645e-8f,  // (d + 16) 2^-24
                          row0, rows, nbr_idx, nbr_d2, kdist2, fallback_rows, fallback_count,
-                         keyrows, centred, norms, 1, n_points};
+                         fallback_cap, keyrows, centred, norms, 1, n_points};
     if (!dispatch(stride, k, &args, stream)) throw std::invalid_argument("sweep_knn: unsupported (n_dims, k)");
     CULOF_CHECK(hipGetLastError());
 }
@@ -477,14 +482,15 @@ void sweep_knn(const float* keyrows, const float* centred, int stride, const flo
 void sweep_query_knn(const float* qkeys, const float* qcentred, const float* qnorms, int n_queries,
                      const float* keyrows, const float* centred, int stride, const float* norms,
                      const int* guard, int n_points, int n_dims, int k, int* nbr_idx, float* nbr_d2,
-                     float* kdist2, int* fallback_rows, int* fallback_count, hipStream_t stream) {
+                     float* kdist2, int* fallback_rows, int* fallback_count, int fallback_cap,
+                     hipStream_t stream) {
     if (stride != padded_dims(n_dims)) throw std::invalid_argument("sweep_query_knn: row stride mismatch");
     if (n_queries <= 0) return;
     ProfileScope prof("sweep_knn", stream);
     const SweepArgs args{keyrows, centred, norms, guard, n_points, k,
                          static_cast<float>(n_dims + 16) * 5.9604645e-8f,  // (d + 16) 2^-24
                          0, n_queries, nbr_idx, nbr_d2, kdist2, fallback_rows, fallback_count,
-                         qkeys, qcentred, qnorms, 0, n_queries};
+                         fallback_cap, qkeys, qcentred, qnorms, 0, n_queries};
     if (!dispatch(stride, k, &args, stream)) throw std::invalid_argument("sweep_query_knn: unsupported (n_dims, k)");
     CULOF_CHECK(hipGetLastError());
 }

@@ -84,7 +84,7 @@ tie_split_kernel(const int* __restrict__ idx1, const float* __restrict__ d21, in
             band.bound[at] = bound;
             band.tband[i] = float_at_least(bound);
         } else {
-            tie_rows[atomicAdd(tie_count, 1)] = i;
+            queue_push(tie_rows, tie_count, rows, i);
         }
         return;
     }
@@ -154,14 +154,14 @@ __device__ void band_clear_from(Band& b, int from) {
 __global__ void __launch_bounds__(kBlock)
 tie_resolve_kernel(const float* __restrict__ xs, int stride, int n, int d, int k, double rho,
                    const int* __restrict__ idx1, const float* __restrict__ d21,
-                   const int* __restrict__ tie_rows, const int* __restrict__ tie_count,
+                   const int* __restrict__ tie_rows, const int* __restrict__ tie_count, int tie_cap,
                    int* __restrict__ nbr_idx, float* __restrict__ nbr_d2,
                    float* __restrict__ kdist2) {
     __shared__ Band b;
     __shared__ double s_bound;
     __shared__ __attribute__((aligned(16))) float s_q[kMaxTieStride];
     const size_t ns = static_cast<size_t>(n);
-    const int queued = *tie_count;
+    const int queued = min(*tie_count, tie_cap);
     for (int q = blockIdx.x; q < queued; q += gridDim.x) {
         const int i = tie_rows[q];
         if (threadIdx.x == 0) {
@@ -263,21 +263,21 @@ void tie_break_rows(const float* keyrows, int stride, int n, int d, int k, int r
         throw std::invalid_argument("FP64 tie-break supports n_dims <= 1024");
     const double rho = tie_band_ratio(d);
     ProfileScope prof("tie_break", s);
-    CULOF_CHECK(hipMemsetAsync(tie_count, 0, sizeof(int), s));
+    zero_fill(tie_count, sizeof(int), s);
     TieBand queue{};
     if (band && band->cap > 0) {
         queue = *band;
-        CULOF_CHECK(hipMemsetAsync(queue.count, 0, sizeof(int), s));
+        zero_fill(queue.count, sizeof(int), s);
     }
     tie_split_kernel<<<(rows + kBlock - 1) / kBlock, kBlock, 0, s>>>(
         idx1, d21, n, row0, rows, k, rho, nbr_idx, nbr_d2, kdist2, tie_rows, tie_count, queue);
     CULOF_CHECK(hipGetLastError());
     if (queue.rows) {
-        tie_band_select(keyrows, stride, n, k, queue, nbr_idx, nbr_d2, kdist2, tie_rows, tie_count, s);
+        tie_band_select(keyrows, stride, n, k, queue, nbr_idx, nbr_d2, kdist2, tie_rows, tie_count, rows, s);
     }
     const int grid = rows < 1024 ? rows : 1024;
     tie_resolve_kernel<<<grid, kBlock, 0, s>>>(keyrows, stride, n, d, k, rho, idx1, d21, tie_rows,
-                                               tie_count, nbr_idx, nbr_d2, kdist2);
+                                               tie_count, rows, nbr_idx, nbr_d2, kdist2);
     CULOF_CHECK(hipGetLastError());
 }
 

@@ -47,7 +47,7 @@
  *
  * The reference reduces each feature with 256 partial sums, partial p holding
  * rows p, p+256, p+512, ... in ascending order, followed by a halving tree
- * over the partials (preprocess.cu:155-164). That order is restated exactly so
+ * over the partials (preprocess.cu:35-45). That order is restated exactly so
  * the FP64 mean is bit-identical to the reference's.
  * ------------------------------------------------------------------------- */
 
@@ -70,7 +70,7 @@ void oracle_center(const float* x, int n, int d, int normalize, float* xc, float
         const double mean = tree_reduce(part) / n;
         double inv_std = 1.0;
         if (normalize) {
-            /* Population variance; preprocess.cu:168-186. The squared
+            /* Population variance; preprocess.cu:48-65. The squared
              * deviation is accumulated with a fused multiply-add. */
             for (int p = 0; p < REDUCE_LANES; ++p) {
                 double s = 0.0;
@@ -90,7 +90,7 @@ void oracle_center(const float* x, int n, int d, int normalize, float* xc, float
             xc[o] = (float)(((double)x[o] - mean) * inv_std);
         }
     }
-    /* preprocess.cu:195-204: FP64 sum of exact float*float products. */
+    /* preprocess.cu:75-84: FP64 sum of exact float*float products. */
     for (int i = 0; i < n; ++i) {
         double s = 0.0;
         const float* row = xc + (size_t)i * d;

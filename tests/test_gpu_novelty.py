@@ -31,6 +31,10 @@ def data(n, m, d, seed):
     (8000, 700, 16, 25, False, _native.FORM_BF16),
     (2500, 300, 128, 50, False, _native.FORM_PLAIN),
     (1500, 200, 3, 1, False, _native.FORM_PLAIN),
+    # more queries than fitted rows: every query block must still visit each
+    # candidate chunk once (its start chunk is taken modulo the chunk count)
+    (1000, 5000, 8, 20, False, _native.FORM_PLAIN),
+    (600, 4100, 32, 10, True, _native.FORM_PLAIN),
 ])
 def test_novelty_bit_exact_with_oracle(n, m, d, k, normalize, form):
     X, Q = data(n, m, d, seed=n + m + d)

@@ -294,6 +294,59 @@ def test_overflow_guard_in_symmetric_pass(form):
         _assert_same_bits(gk[r], ok[0], f"row {r} kdist2")
 
 
+def test_overflow_guard_with_wide_outliers_at_scale():
+    """The guard at n > 65,536 on data with wide outliers (the reference's
+    benchmark distribution, whose outliers would otherwise be queued for the
+    list pass as heavy rows): every row goes through the radix redo exactly
+    once, and sampled rows, outliers included, match the oracle."""
+    from _gpu import knn
+
+    from culof_amd import _native
+    from culof_amd.benchmark import blobs_with_outliers
+
+    n, d, k = 100_000, 8, 20
+    X = blobs_with_outliers(n, d, seed=17) * np.float32(1.2e17)  # keys stay below FLT_MAX
+    X[5] = np.float32(20 * 1.2e17)                                 # a norm past FLT_MAX / 16
+    print(f"guard at scale: n={n} d={d} k={k}", flush=True)
+    _native.profile_reset()
+    _native.profile_enable(True)
+    try:
+        gi, gd, gk = knn(X, k)
+    finally:
+        _native.profile_enable(False)
+    assert _native.profile_read("radix_knn")[0] > 5 * _native.profile_read("mfma_collect")[0]
+    rng = np.random.default_rng(5)
+    for r in [5, 0, 1, 2, *rng.choice(n, 6, replace=False).tolist()]:
+        oi, od, ok = oracle.knn_rows(X, k, int(r), 1)
+        np.testing.assert_array_equal(gi[:, r], oi[0], err_msg=f"row {r}")
+        _assert_same_bits(gd[:, r], od[0], f"row {r} nbr_d2")
+        _assert_same_bits(gk[r], ok[0], f"row {r} kdist2")
+
+
+def test_small_scratch_budget_tiles_rows(monkeypatch):
+    """CULOF_SCRATCH_BUDGET below the full run's collect buffers splits the
+    query rows into tiles sized to the budget (the reference's tile_rows_for,
+    src/cuda/pipeline.cu:99-110); the result is the same bits."""
+    from _gpu import forward
+
+    from culof_amd import _native
+
+    X = make_data(50_000, 32, seed=23, n_outliers=500)
+    print("budget tiling: n=50000 d=32 k=20", flush=True)
+    want = forward(X, 20)
+    monkeypatch.setenv("CULOF_SCRATCH_BUDGET", str(1 << 30))
+    _native.profile_reset()
+    _native.profile_enable(True)
+    try:
+        got = forward(X, 20)
+    finally:
+        _native.profile_enable(False)
+    launches = _native.profile_read("mfma_collect")[1]
+    assert launches >= 2, f"expected several row tiles under a 1 GiB budget, got {launches}"
+    for key in ("scores", "idx", "d2", "kdist2", "lrd"):
+        _assert_same_bits(got[key], want[key], key)
+
+
 def test_every_fallback_tier():
     """10,000 copies of one point among 150,000: their collected sets overflow
     the first pass, the second pass and the list queue, so the rows run
