@@ -33,6 +33,7 @@ SIGNATURES = {
     "culof_device_info": (_c_int, [_cp, _sz]),
     "culof_lof": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp]),
     "culof_lof_device": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "culof_lof_device_noalloc": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "culof_lof_novelty_device": (_c_int, [_vp, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp,
                                           _vp, _vp, _vp]),
     "culof_lof_backward_device": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),

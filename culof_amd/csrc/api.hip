@@ -24,6 +24,7 @@
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <sstream>
 #include <stdexcept>
@@ -148,10 +149,15 @@ size_t scratch_budget_env() {
 }
 
 // A call's device scratch: stream-ordered allocations (hipMallocAsync on the
-// call's stream from the default pool), freed when the call returns.
+// call's stream from the default pool), freed when the call returns; or, for
+// the no-allocation entry point, consecutive pieces of one fixed device
+// region, where running past the region's end is an argument error and no
+// allocator is called.
 class Scratch {
 public:
     explicit Scratch(hipStream_t s) : stream_(s), budget_(scratch_budget_env()) {}
+    Scratch(hipStream_t s, void* region, size_t bytes)
+        : stream_(s), region_(static_cast<unsigned char*>(region)), cap_(bytes) {}
     ~Scratch() {
         for (void* p : ptrs_) (void)hipFreeAsync(p, stream_);
     }
@@ -167,6 +173,17 @@ public:
     template <typename T>
     T* get(size_t count) {
         const size_t bytes = footprint<T>(count);
+        if (region_) {
+            if (bytes > cap_ - used_) {
+                std::ostringstream os;
+                os << "the call needs more than the " << (cap_ >> 20)
+                   << " MiB of scratch the no-allocation entry point holds";
+                throw std::invalid_argument(os.str());
+            }
+            void* p = region_ + used_;
+            used_ += bytes;
+            return static_cast<T*>(p);
+        }
         void* p = nullptr;
         CULOF_CHECK(hipMallocAsync(&p, bytes, stream_));
         ptrs_.push_back(p);
@@ -177,13 +194,19 @@ public:
     // Bytes later get() calls may still take (free device memory and the
     // pool's idle blocks less the margin, within CULOF_SCRATCH_BUDGET).
     size_t available() const {
+        if (region_) return cap_ - used_;
         size_t avail = device_scratch_available();
         if (budget_) avail = std::min(avail, budget_ > used_ ? budget_ - used_ : size_t{0});
         return avail;
     }
 
+    // True for a fixed region: the pipeline then takes its low-memory paths.
+    bool fixed() const { return region_ != nullptr; }
+
 private:
     hipStream_t stream_;
+    unsigned char* region_ = nullptr;
+    size_t cap_ = 0;
     size_t budget_ = 0;
     size_t used_ = 0;
     std::vector<void*> ptrs_;
@@ -324,14 +347,16 @@ void knn_rows(const Prepared& p, int n, int d, int k, int form, int row0, int ro
         int* idx1 = scratch.get<int>(n1);
         float* d21 = scratch.get<float>(n1);
         float* kd1 = scratch.get<float>(static_cast<size_t>(n));
-        knn_rows(p, n, d, k + 1, screen, row0, rows, method, idx1, d21, kd1, scratch, s);
         int* tie_rows = scratch.get<int>(static_cast<size_t>(rows));
         int* tie_count = scratch.get<int>(1);
+        knn_rows(p, n, d, k + 1, screen, row0, rows, method, idx1, d21, kd1, scratch, s);
         // Near-tie rows are collected on the matrix cores like the main pass
-        // when that is the path serving it; the brute-force scan takes the rest.
+        // when that is the path serving it (and scratch can be allocated); the
+        // brute-force scan takes the rest.
         TieBand band{};
         const int cap = tie_band_capacity(n, d, p.stride, screen, rows);
-        if (method == 0 && cap > 0 && choose_path(d, k + 1, screen, p.stride) == Path::kMfma) {
+        if (method == 0 && cap > 0 && !scratch.fixed() &&
+            choose_path(d, k + 1, screen, p.stride) == Path::kMfma) {
             band = TieBand{p.xc, p.norms, p.guard, screen, cap,
                            scratch.get<int>(static_cast<size_t>(cap)), scratch.get<int>(1),
                            scratch.get<double>(static_cast<size_t>(cap)),
@@ -345,7 +370,11 @@ void knn_rows(const Prepared& p, int n, int d, int k, int form, int row0, int ro
     Path path = Path::kRadix;
     if (method == 0) path = choose_path(d, k, form, p.stride);
     if (method == 2) path = Path::kValu;
-    const bool order = path == Path::kMfma && ordering_enabled() && order_supported(p.stride);
+    // Candidate order and seeds feed the collect pipeline, whose buffers grow
+    // with the rows; a fixed scratch region takes the list sweep instead
+    // (no order, per-row lists in registers; the same bits).
+    const bool order = path == Path::kMfma && ordering_enabled() && order_supported(p.stride) &&
+                       !scratch.fixed();
     // Only the matrix-core path keeps per-row scratch (its collect buffers).
     auto tile_bytes = [&](int t) {
         if (path != Path::kMfma) return size_t{0};
@@ -399,13 +428,9 @@ void knn_rows(const Prepared& p, int n, int d, int k, int form, int row0, int ro
     radix_knn(p.xs, p.stride, n, d, k, 0, rows, fb_rows, fb_count, nbr_idx, nbr_d2, kdist2, s);
 }
 
-void forward(const float* points, int n, int d, int k, bool normalize, int form, float* scores,
-             int* nbr_idx, float* nbr_d2, float* kdist2, float* lrd, hipStream_t s) {
-    if (screen_form(form) < 0) throw std::invalid_argument("unknown precision form");
-    int device = 0;
-    CULOF_CHECK(hipGetDevice(&device));
-    enable_pool_reuse(device);
-    Scratch scratch(s);
+void forward_with(const float* points, int n, int d, int k, bool normalize, int form, float* scores,
+                  int* nbr_idx, float* nbr_d2, float* kdist2, float* lrd, Scratch& scratch,
+                  hipStream_t s) {
     const size_t nn = static_cast<size_t>(n);
     const size_t nk = nn * static_cast<size_t>(k);
     if (!nbr_idx) nbr_idx = scratch.get<int>(nk);
@@ -417,6 +442,48 @@ void forward(const float* points, int n, int d, int k, bool normalize, int form,
     compute_lrd(nbr_idx, nbr_d2, kdist2, n, k, lrd, s);
     compute_scores(nbr_idx, lrd, n, k, scores, s);
 }
+
+void forward(const float* points, int n, int d, int k, bool normalize, int form, float* scores,
+             int* nbr_idx, float* nbr_d2, float* kdist2, float* lrd, hipStream_t s) {
+    if (screen_form(form) < 0) throw std::invalid_argument("unknown precision form");
+    int device = 0;
+    CULOF_CHECK(hipGetDevice(&device));
+    enable_pool_reuse(device);
+    Scratch scratch(s);
+    forward_with(points, n, d, k, normalize, form, scores, nbr_idx, nbr_d2, kdist2, lrd, scratch, s);
+}
+
+// Least scratch forward_with takes from a fixed region: every buffer it
+// carves there, with the kNN rows in tiles of at most kMinTileRows.
+size_t region_floor(const float* points, int n, int d, int k, bool normalize, int form,
+                    bool have_outputs) {
+    const size_t nn = static_cast<size_t>(n);
+    const int screen = screen_form(form);
+    const int stride = mfma_padded_dims(d, screen);
+    const size_t cells = nn * static_cast<size_t>(stride);
+    size_t b = 0;
+    if (!have_outputs) {
+        b += 2 * Scratch::footprint<int>(nn * static_cast<size_t>(k)) + 2 * Scratch::footprint<float>(nn);
+    }
+    const bool in_place = !normalize && stride == d && reinterpret_cast<uintptr_t>(points) % 16 == 0;
+    b += Scratch::footprint<float>(cells) + Scratch::footprint<float>(nn) +
+         Scratch::footprint<double>(static_cast<size_t>(kPrepWork) * d) + Scratch::footprint<int>(1);
+    if (!normalize && !in_place) b += Scratch::footprint<float>(cells);
+    int k_sel = k;
+    if ((form & kFormTiesFp64) && k + 1 <= n - 1) {
+        k_sel = k + 1;
+        b += 2 * Scratch::footprint<int>(nn * static_cast<size_t>(k_sel)) + 2 * Scratch::footprint<int>(nn) +
+             Scratch::footprint<int>(1);
+    }
+    const Path path = choose_path(d, k_sel, screen, stride);
+    if (path != Path::kRadix) b += Scratch::footprint<int>(1) + Scratch::footprint<int>(nn);
+    if (path == Path::kMfma) {
+        b += Scratch::footprint<unsigned char>(
+            mfma_scratch_bytes(n, stride, screen, std::min(n, kMinTileRows), k_sel, false));
+    }
+    return b;
+}
+
 
 // New points scored against a fitted set (scikit-learn's novelty mode,
 // LocalOutlierFactor(novelty=True).score_samples): the forward on `points`
@@ -534,6 +601,33 @@ int guarded(F&& f) {
 hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
 }  // namespace
+
+int run_guarded(const std::function<void()>& f) { return guarded(f); }
+
+// The forward of culof_lof_device with all scratch in [region, region + bytes)
+// (culof_lof_device_noalloc): arguments checked, the region's size checked
+// against the least the call needs before anything is launched, no allocator
+// called. Calls that share a region must not overlap.
+void forward_in_region(const float* points, int n, int d, int k, bool normalize, int form,
+                       float* scores, int* nbr_idx, float* nbr_d2, float* kdist2, float* lrd,
+                       void* region, size_t bytes, hipStream_t s) {
+    validate(points, n, d, k);
+    if (!scores) throw std::invalid_argument("scores must not be null");
+    if (screen_form(form) < 0) throw std::invalid_argument("unknown precision form");
+    require_device();
+    const bool have_outputs = nbr_idx && nbr_d2 && kdist2 && lrd;
+    const size_t need = region_floor(points, n, d, k, normalize, form, have_outputs);
+    if (need > bytes) {
+        std::ostringstream os;
+        os << "n_points=" << n << ", n_dims=" << d << ", k=" << k << " needs at least "
+           << ((need + (1 << 20) - 1) >> 20) << " MiB of scratch; the no-allocation entry point holds "
+           << (bytes >> 20) << " MiB (use culof_lof_device)";
+        throw std::invalid_argument(os.str());
+    }
+    Scratch scratch(s, region, bytes);
+    forward_with(points, n, d, k, normalize, form, scores, nbr_idx, nbr_d2, kdist2, lrd, scratch, s);
+}
+
 }  // namespace detail
 
 // ---------------------------------------------------------------------------

@@ -10,6 +10,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <string>
 
 #include "culof.h"
@@ -222,6 +223,16 @@ void compute_query_scores(const int* nbr_idx, const float* lrd_fit, const float*
 
 // Same arithmetic as the reference's choose_tile_rows (src/cuda/pipeline.cu:33-38).
 int choose_tile_rows(int n_points, size_t budget_bytes);
+
+// ---- api.hip: pieces the no-allocation entry point (noalloc.hip) shares ----
+// Runs f, turning its exceptions into a culof_status and culof_last_error().
+int run_guarded(const std::function<void()>& f);
+// culof_lof_device with every scratch buffer taken from [region, region +
+// bytes) and no allocator called; a call needing more is rejected before
+// anything is launched.
+void forward_in_region(const float* points, int n, int d, int k, bool normalize, int form,
+                       float* scores, int* nbr_idx, float* nbr_d2, float* kdist2, float* lrd,
+                       void* region, size_t bytes, hipStream_t s);
 
 }  // namespace detail
 }  // namespace culof

@@ -82,6 +82,27 @@ int culof_lof_device(const float* points, int n_points, int n_dims, int k, int n
                      int form, float* scores, int* nbr_idx, float* nbr_d2, float* kdist2,
                      float* lrd, void* stream);
 
+/* Bytes of device scratch culof_lof_device_noalloc holds per device. */
+#define CULOF_NOALLOC_SCRATCH_BYTES (256u * 1024u * 1024u)
+
+/* culof_lof_device without any device allocation: same arguments, checks and
+ * results (bit-identical), with every scratch buffer taken from a fixed
+ * array of CULOF_NOALLOC_SCRATCH_BYTES that each device holds from the first
+ * call on it until the process ends. No counterpart in the reference; it is
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). It calls no hipMalloc,
+ * hipMallocAsync or hipFree. A call whose scratch would exceed the array
+ * returns CULOF_INVALID_ARGUMENT before launching anything: at most
+ * 4 (3 p + 4 k + 16) bytes per point with p the padded n_dims (4, 8, 16, 32,
+ * 64, 128, or n_dims rounded up to 4), plus about 2 MiB, e.g. 550k points at
+ * n_dims = 8, k = 20. Every call on a device shares that device's array: calls that overlap in time on one device
+ * (on different streams or threads) race and must be serialised by the
+ * caller. It takes the list sweep rather than the collect pipeline, so it is
+ * slower than culof_lof_device on large inputs. Never synchronises. */
+int culof_lof_device_noalloc(const float* points, int n_points, int n_dims, int k, int normalize,
+                             int form, float* scores, int* nbr_idx, float* nbr_d2, float* kdist2,
+                             float* lrd, void* stream);
+
 /* New points scored against a fitted set. No counterpart in the reference
  * (its estimator has no novelty mode, README.md:29-34); this is scikit-learn's
  * LocalOutlierFactor(novelty=True).fit(points).score_samples(queries) with the
