@@ -29,14 +29,20 @@ def _split(total: int, parts: int) -> list[int]:
 
 
 def clusters(n: int = 200_000, d: int = 8, n_clusters: int = 5, spread: float = 10.0,
-             outlier_box: float = 15.0, outlier_frac: float = 0.01, seed: int = 0) -> np.ndarray:
+             outlier_box: float = 15.0, outlier_frac: float = 0.01, seed: int = 0,
+             return_outliers: bool = False):
+    """X, or (X, mask of the planted uniform outlier rows) with ``return_outliers``."""
     rng = np.random.default_rng(seed)
     n_out = int(round(n * outlier_frac))
     centres = rng.uniform(-spread, spread, size=(n_clusters, d))
     blocks = [centres[c] + rng.standard_normal((m, d)) for c, m in enumerate(_split(n - n_out, n_clusters))]
     blocks.append(rng.uniform(-outlier_box, outlier_box, size=(n_out, d)))
     X = np.concatenate(blocks).astype(np.float32)
-    return np.ascontiguousarray(X[rng.permutation(n)])
+    perm = rng.permutation(n)
+    X = np.ascontiguousarray(X[perm])
+    if return_outliers:
+        return X, perm >= n - n_out
+    return X
 
 
 def lowrank(n: int = 100_000, d: int = 128, rank: int = 32, n_clusters: int = 16,
