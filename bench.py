@@ -9,16 +9,24 @@ culof_lof_device. W untimed steps, then K timed steps bracketed by
 torch.cuda.synchronize(). Prints one JSON line:
 
 * value        points/s over the K timed steps (n * K / seconds)
-* roofline     the dominant kernel (the first collect pass): algorithmic FP32
-               flops of the distance contraction (2 d flops per query-candidate
-               pair, n x n pairs per forward) over its average launch time,
-               measured with HIP events on the stream it runs on, beside the
-               matrix-core flops it executed (each unordered pair once);
-               traffic from the committed rocprofv3 counter summary when present
+* roofline     the dominant kernel (the first collect pass, on the FP32 matrix
+               cores): the matrix-core flops it executes per launch (2 d per
+               pair of every 128-row block pair the symmetric pass visits, each
+               unordered pair once) over its average launch time, measured with
+               HIP events on the stream it runs on, against the FP32 matrix
+               peak; beside it the effective rate that counts the whole n x n
+               contraction (2 d n^2, the reference's Gram tiles); traffic from
+               the committed rocprofv3 counter summary
+* hbm          the select and scatter kernels: HBM bytes per launch from the
+               committed FETCH_SIZE / WRITE_SIZE summary (calibrated by a
+               plain copy) over their live average launch time, against the
+               8 TB/s HBM peak
 * cpu_baseline the CPU oracle (a single-threaded C port of the same path)
                timed on this host over a bounded sample of query rows
 * sklearn      scikit-learn's LocalOutlierFactor on all host cores over the
-               full workload (the reference's own comparator), with agreement
+               full workload, fitted on float32 input (the reference's own
+               comparator and timing) and on float64 input (its FP64 path, the
+               agreement target), with agreement against both
 * parity       the sampled rows' neighbour lists and k-distances compared bit
                for bit with the oracle, and scores against scikit-learn
 * host_path    culof_amd.lof on the host array, best of 7 (PCIe copies
@@ -54,9 +62,12 @@ FP32_VALU_TFLOPS = 78.6       # non-packed FP32 VALU (packed FP32 ops are disabl
 HBM_PEAK_GBS = 8000.0
 REFERENCE_PTS_S = 1.42e5      # reference on RTX A6000 at n=200k, d=8, k=20 (its README.md:50)
 KERNELS = ("center_partials", "center_apply", "order", "seed", "heavy_rows", "screen_prep",
-           "mfma_collect", "scatter_pairs", "second_collect", "collect_select", "mfma_sweep",
-           "merge_slices", "sweep_knn", "radix_knn", "tie_break", "tie_collect", "tie_band", "lrd",
-           "score")
+           "mfma_collect", "scatter_pairs", "collect_select", "big_select",
+           "second_collect",
+           "second_select", "mfma_sweep", "merge_slices", "sweep_knn", "radix_knn", "tie_break",
+           "tie_collect", "tie_band", "lrd", "score")
+# Timing scope -> kernel name in the rocprofv3 counter summary.
+HBM_KERNELS = {"collect_select": "collect_select_kernel<8>", "scatter_pairs": "scatter_pairs_kernel"}
 
 
 def log(msg: str) -> None:
@@ -95,13 +106,48 @@ def calibrate_copy(mib: int) -> int:
     return 0
 
 
+def _counter_summary(kernel: str, workload: str):
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_counters.md")
+    return open(path).read() if os.path.exists(path) else None
+
+
 def read_traffic(kernel: str, workload: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_counters.md")
-    if not os.path.exists(path):
-        return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    text = _counter_summary(kernel, workload)
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text) if text else None
     return float(m.group(1)) if m else None
+
+
+def kernel_bytes(summary_kernel: str, kernel: str, workload: str):
+    """HBM bytes per dispatch of `kernel` in the counter summary of
+    `summary_kernel`: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) times the
+    copy-calibrated scale factors the summary states."""
+    text = _counter_summary(summary_kernel, workload)
+    if not text:
+        return None
+    factor = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        m = re.search(rf"\|\s*{counter}\s*\|[^|]*\|\s*([0-9.]+)\s*\|\s*$", text, re.M)
+        factor[counter] = float(m.group(1)) if m else None
+    total = 0.0
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        m = re.search(rf"\|\s*`{re.escape(kernel)}`\s*\|\s*{counter}\s*\|\s*\d+\s*\|\s*([0-9.]+)\s*\|",
+                      text)
+        if not m or factor[counter] is None:
+            return None
+        total += float(m.group(1)) * factor[counter] * 1024.0
+    return total
+
+
+def executed_flops(n: int, dp: int, form: int) -> float:
+    """Matrix-core flops (2 dp per pair) one launch of the symmetric collect
+    pass executes: blocks of 128 query rows (256 for the BF16 screen at
+    dp >= 64), each block with itself and the next (NB - 1) / 2 blocks of the
+    ring (plus NB / 2 for half of them when NB is even)."""
+    rows = 256 if (form == 1 and dp >= 64) else 128
+    nb = (n + rows - 1) // rows
+    block_pairs = nb * ((nb - 1) // 2 + 1) + (nb // 2 if nb % 2 == 0 else 0)
+    return 2.0 * dp * block_pairs * float(rows) * float(rows)
 
 
 def _time_device(fn, steps: int) -> float:
@@ -282,18 +328,12 @@ def main() -> int:
     value = n * args.steps / elapsed
     log(f"[bench] {args.steps} steps in {elapsed:.3f} s: {ms_per_step:.3f} ms/step, {value:.4g} points/s")
 
-    # ---- roofline of the dominant kernel (the fused sweep) ----
+    # ---- roofline of the dominant kernel (the symmetric collect pass) ----
     avg_sweep_s = sweep_ms / max(sweep_launches, 1) / 1e3
-    pairs = float(n) * float(n)                  # one launch covers every query row
-    flops = 2.0 * d * pairs
-    achieved = flops / avg_sweep_s / 1e12
-    # The symmetric first pass computes each unordered pair of 32-row tiles
-    # once: blocks of 128 rows, each with its own block and the next
-    # (NB - 1) / 2 blocks (plus NB / 2 for half of them when NB is even).
-    nb = (n + 127) // 128
-    block_pairs = nb * ((nb - 1) // 2 + 1) + (nb // 2 if nb % 2 == 0 else 0)
-    executed = 2.0 * d * block_pairs * 128.0 * 128.0
-    executed_tflops = executed / avg_sweep_s / 1e12
+    dp = -(-d // 4) * 4 if d > 4 else 4
+    executed = executed_flops(n, dp, _native.FORM_PLAIN)
+    achieved = executed / avg_sweep_s / 1e12
+    effective = 2.0 * d * float(n) * float(n) / avg_sweep_s / 1e12
     roofline = {
         "bound": "mfma",
         "achieved": round(achieved, 3),
@@ -303,18 +343,26 @@ def main() -> int:
         "traffic": read_traffic(dominant, args.workload),
         "kernel": dominant,
         "avg_launch_ms": round(avg_sweep_s * 1e3, 4),
-        "executed_tflops": round(executed_tflops, 3),
-        "executed_frac": round(executed_tflops / FP32_PEAK_TFLOPS, 4),
-        "frac_of_valu_fp32_peak": round(achieved / FP32_VALU_TFLOPS, 4),
-        "select_bytes_effective_gbs": round(20.0 * pairs / avg_sweep_s / 1e9, 1),
-        "note": "achieved counts 2*d flops for every ordered query-candidate pair (F = 2 n^2 d, "
-                "the full n x n contraction of the reference's Gram tiles); the symmetric "
-                "collect pass computes each unordered pair once, so achieved is an effective "
-                "rate and executed_tflops / executed_frac give the matrix-core flops it ran "
-                "(2*d per pair of every 128-row block pair it visits); selection never stores "
-                "distances, so select_bytes_effective_gbs (20 B/pair, the stored-tile figure) "
-                "is an equivalent, not HBM traffic",
+        "flops_per_launch": executed,
+        "effective_tflops": round(effective, 3),
+        "effective_frac": round(effective / FP32_PEAK_TFLOPS, 4),
+        "note": "achieved = FP32 matrix-core flops the symmetric collect pass executes per "
+                "launch (2*dp per pair of every 128-row block pair it visits, each unordered "
+                "pair once; flops_per_launch) / its average HIP-event launch time, against the "
+                "FP32 matrix peak; effective_* count 2*d*n^2 (every ordered pair, the reference's "
+                "Gram contraction) over the same time; traffic = HBM bytes per launch from the "
+                "committed FETCH_SIZE/WRITE_SIZE summary",
     }
+    hbm = {}
+    for scope, kname in HBM_KERNELS.items():
+        ms, launches = total_kernel.get(scope, (0.0, 0))
+        nbytes = kernel_bytes(dominant, kname, args.workload)
+        if not launches or nbytes is None:
+            continue
+        avg_s = ms / launches / 1e3
+        hbm[scope] = {"kernel": kname, "avg_launch_ms": round(avg_s * 1e3, 4),
+                      "bytes_per_launch": round(nbytes), "achieved_gbs": round(nbytes / avg_s / 1e9, 1),
+                      "frac_of_hbm_peak": round(nbytes / avg_s / 1e9 / HBM_PEAK_GBS, 4)}
     kernels = {name: {"ms_per_step": round(ms / args.steps, 4), "launches": cnt}
                for name, (ms, cnt) in total_kernel.items() if cnt}
 
@@ -353,20 +401,26 @@ def main() -> int:
         from sklearn.neighbors import LocalOutlierFactor
 
         cores = len(os.sched_getaffinity(0))
-        log(f"[bench] scikit-learn LocalOutlierFactor(n_jobs=-1) on {cores} cores")
-        t = time.perf_counter()
-        sk = -LocalOutlierFactor(n_neighbors=k, n_jobs=-1).fit(X).negative_outlier_factor_
-        sk_s = time.perf_counter() - t
-        rel = np.abs(result_scores - sk) / np.maximum(np.abs(sk), 1e-9)
         top = max(1, n // 100)
-        agree = len(set(np.argsort(-result_scores)[:top]) & set(np.argsort(-sk)[:top])) / top
-        sklearn_leg = {"value": round(n / sk_s, 2), "unit": "points/s", "cores": cores,
-                       "seconds": round(sk_s, 3), "input": "float32 X, as the reference benchmarks it"}
-        parity["vs_sklearn_median_rel"] = float(np.median(rel))
-        parity["vs_sklearn_p99_rel"] = float(np.percentile(rel, 99))
-        parity["vs_sklearn_max_rel"] = float(rel.max())
-        parity["vs_sklearn_top1pct_agreement"] = agree
-        log(f"[bench] sklearn {sk_s:.2f} s; agreement {parity}")
+        sklearn_leg = {"unit": "points/s", "cores": cores}
+        for label, dtype in (("fp32", np.float32), ("fp64", np.float64)):
+            log(f"[bench] scikit-learn LocalOutlierFactor(n_jobs=-1) on {cores} cores, {label} input")
+            Xs = X.astype(dtype)
+            t = time.perf_counter()
+            sk = -LocalOutlierFactor(n_neighbors=k, n_jobs=-1).fit(Xs).negative_outlier_factor_
+            sk_s = time.perf_counter() - t
+            rel = np.abs(result_scores - sk) / np.maximum(np.abs(sk), 1e-9)
+            agree = len(set(np.argsort(-result_scores)[:top]) & set(np.argsort(-sk)[:top])) / top
+            sklearn_leg[label] = {"value": round(n / sk_s, 2), "seconds": round(sk_s, 3)}
+            parity[f"vs_sklearn_{label}_median_rel"] = float(np.median(rel))
+            parity[f"vs_sklearn_{label}_p99_rel"] = float(np.percentile(rel, 99))
+            parity[f"vs_sklearn_{label}_max_rel"] = float(rel.max())
+            parity[f"vs_sklearn_{label}_top1pct_agreement"] = agree
+            log(f"[bench] sklearn ({label} input) {sk_s:.2f} s")
+        sklearn_leg["value"] = sklearn_leg["fp32"]["value"]
+        sklearn_leg["note"] = ("value: float32 input, the reference's benchmark convention; "
+                               "fp64: float64 input (scikit-learn's FP64 path, the accuracy target)")
+        log(f"[bench] agreement {parity}")
 
     # ---- host arrays in and out (the reference's own timing convention:
     # culof.lof on a numpy array, best of 7 after a warm-up); never `value` ----
@@ -404,6 +458,7 @@ def main() -> int:
         "config": {"workload": args.workload, "n": n, "d": d, "k": k, "seed": args.seed,
                    "generator": spec["gen"], "device_resident_input": True},
         "roofline": roofline,
+        "hbm": hbm,
         "cpu_baseline": cpu_baseline,
         "sklearn": sklearn_leg,
         "host_path": host_path,

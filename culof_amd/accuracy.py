@@ -139,20 +139,35 @@ class AccuracyReport:
         return "\n".join(lines)
 
 
+def sklearn_scores(X: np.ndarray, k: int, dtype=np.float64) -> np.ndarray:
+    """LOF of scikit-learn's LocalOutlierFactor fitted on ``X`` cast to ``dtype``.
+
+    Fitted on float64 input, scikit-learn keeps its kNN distances and
+    densities in FP64; fitted on float32 (as the reference's own comparisons
+    do, tests/python/conftest.py:10-12) it casts the distances to float32
+    first (sklearn/neighbors/_lof.py:294-298), a different comparator."""
+    from sklearn.neighbors import LocalOutlierFactor
+
+    est = LocalOutlierFactor(n_neighbors=k, n_jobs=-1)
+    return -est.fit(np.asarray(X, dtype=dtype)).negative_outlier_factor_
+
+
 def report(X: np.ndarray, k: int, scores: np.ndarray | None = None, form: str = "fp32",
            ties: str = "index", threshold: float = 1e-4,
-           reference: np.ndarray | None = None) -> AccuracyReport:
+           reference: np.ndarray | None = None, comparator: str = "fp64") -> AccuracyReport:
     """Accuracy report of culof_amd (or of ``scores``) on ``X`` against
-    scikit-learn's LocalOutlierFactor in FP64 (or against ``reference``)."""
+    scikit-learn's LocalOutlierFactor fitted on float64 input (``comparator``
+    "fp32": on float32 input, the reference's comparator), or against
+    ``reference``."""
     X = np.ascontiguousarray(X, dtype=np.float32)
     if scores is None:
         from . import lof
 
         scores = lof(X, k, form=form, ties=ties)
     if reference is None:
-        from sklearn.neighbors import LocalOutlierFactor
-
-        reference = -LocalOutlierFactor(n_neighbors=k).fit(X).negative_outlier_factor_
+        if comparator not in ("fp64", "fp32"):
+            raise ValueError(f"comparator must be 'fp64' or 'fp32', got {comparator!r}")
+        reference = sklearn_scores(X, k, np.float64 if comparator == "fp64" else np.float32)
     stats = score_stats(scores, reference, threshold)
     rel = relative_errors(scores, reference)
     above = np.flatnonzero(rel > threshold)
@@ -178,10 +193,14 @@ def main(argv: list[str] | None = None) -> int:
     p.add_argument("--form", choices=("fp32", "bf16"), default="fp32")
     p.add_argument("--ties", choices=("index", "fp64"), default="index")
     p.add_argument("--threshold", type=float, default=1e-4)
+    p.add_argument("--comparator", choices=("fp64", "fp32"), default="fp64",
+                   help="scikit-learn fitted on float64 input (default) or on float32 input "
+                        "(the comparator of the reference's published accuracy figures)")
     p.add_argument("--json", type=str, default=None)
     args = p.parse_args(argv)
     X = blobs_with_outliers(args.n, args.d, seed=args.seed)
-    rep = report(X, args.k, form=args.form, ties=args.ties, threshold=args.threshold)
+    rep = report(X, args.k, form=args.form, ties=args.ties, threshold=args.threshold,
+                 comparator=args.comparator)
     print(rep.text())
     if args.json:
         with open(args.json, "w") as fh:

@@ -1764,13 +1764,17 @@ void launch_collect(const MfmaArgs& a, hipStream_t stream) {
         ProfileScope prof(a.big_rows ? "mfma_collect" : "second_collect", stream);
         mfma_collect_kernel<Op, DP><<<grid, 64 * kTilesPerBlock, 0, stream>>>(a);
     }
-    ProfileScope prof("collect_select", stream);
     if (a.big_rows) {
         // First pass: a wave per row, large sets to the block select.
-        collect_select_kernel<DP><<<(a.rows + 3) / 4, 256, 0, stream>>>(a);
+        {
+            ProfileScope prof("collect_select", stream);
+            collect_select_kernel<DP><<<(a.rows + 3) / 4, 256, 0, stream>>>(a);
+        }
+        ProfileScope prof("big_select", stream);
         big_select_kernel<DP><<<big_grid, kBigThreads, 0, stream>>>(a, a.big_rows, a.big_count);
     } else {
         // Second pass over overflowed rows: all large.
+        ProfileScope prof("second_select", stream);
         big_select_kernel<DP><<<big_grid, kBigThreads, 0, stream>>>(a, nullptr, a.row_count);
     }
 }
@@ -1807,8 +1811,11 @@ void launch_sym(const MfmaArgs& a, const SymBuffers& sb, hipStream_t stream) {
                                                       a.guard, a.cbuf, a.ccount);
         mark_overflow_kernel<<<(a.n + 255) / 256, 256, 0, stream>>>(sb.col_over, a.n, sb.slices, a.cap, a.ccount);
     }
-    ProfileScope prof("collect_select", stream);
-    collect_select_kernel<DP><<<(a.rows + 3) / 4, 256, 0, stream>>>(a);
+    {
+        ProfileScope prof("collect_select", stream);
+        collect_select_kernel<DP><<<(a.rows + 3) / 4, 256, 0, stream>>>(a);
+    }
+    ProfileScope prof("big_select", stream);
     big_select_kernel<DP><<<std::min(a.rows, 1024), kBigThreads, 0, stream>>>(a, a.big_rows, a.big_count);
 }
 

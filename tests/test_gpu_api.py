@@ -33,7 +33,8 @@ def points(n: int, d: int, seed: int, n_outliers: int = 0) -> np.ndarray:
 
 
 def sk(X, k):
-    return -LocalOutlierFactor(n_neighbors=k).fit(X).negative_outlier_factor_
+    """scikit-learn's LOF fitted on float64 input (its FP64 path)."""
+    return -LocalOutlierFactor(n_neighbors=k).fit(np.asarray(X, np.float64)).negative_outlier_factor_
 
 
 def max_rel(got, want):
@@ -175,3 +176,34 @@ def test_env_path_switches_give_the_same_bits(monkeypatch):
         np.testing.assert_array_equal(lof(X, 20).view(np.uint32), base.view(np.uint32))
         monkeypatch.delenv(var)
     assert os.environ.get("CULOF_KNN_PATH") is None
+
+
+@pytest.mark.parametrize("scale", [1e-6, 1.0, 1e3, 1e9])
+def test_scale_regression_through_the_hip_path(scale):
+    """The reference's overflow-guard regression (tests/python/test_regressions.py:140-146)
+    on the HIP path: 64 x 4 points at scales 1e-6 ... 1e9 give finite scores,
+    bit-exact with the oracle; and the same scales at n = 40,000, d = 8 (the
+    symmetric matrix-core pass, whose screen margins scale with the norms)
+    keep sampled rows bit-exact with the oracle in both screen forms."""
+    import oracle
+    from _gpu import forward
+
+    from culof_amd import _native
+
+    rng = np.random.default_rng(0)
+    X = (rng.standard_normal((64, 4)) * scale).astype(np.float32)
+    s = lof(X, 5)
+    assert np.isfinite(s).all(), f"scale {scale} produced non-finite scores"
+    np.testing.assert_array_equal(s.view(np.uint32), oracle.lof(X, 5).view(np.uint32))
+
+    big = (points(40_000, 8, seed=12, n_outliers=400) * np.float32(scale)).astype(np.float32)
+    print(f"scale {scale}: n=40000 d=8 k=20", flush=True)
+    rows = [0, 1, 399, *np.random.default_rng(1).choice(40_000, 8, replace=False).tolist()]
+    want = {r: oracle.knn_rows(big, 20, r, 1) for r in rows}
+    for form in (_native.FORM_PLAIN, _native.FORM_BF16):
+        got = forward(big, 20, form=form)
+        assert np.isfinite(got["scores"]).all()
+        for r in rows:
+            oi, od, ok = want[r]
+            np.testing.assert_array_equal(got["idx"][:, r], oi[0], err_msg=f"scale {scale} row {r}")
+            assert got["kdist2"][r].view(np.uint32) == ok[0].view(np.uint32), f"scale {scale} row {r}"

@@ -19,14 +19,17 @@ pytestmark = pytest.mark.gpu
 
 
 def test_sweeps_agree_with_sklearn():
-    rows = bm.run_sweeps(k=20, repeats=1, max_n=5000, n_jobs=16, sizes=(1_000, 5_000),
-                         dims=(2, 8, 32, 128), n_dims=5_000)
+    """Both of the reference's sweeps up to n = 20k, the sizes of its
+    published table rows (README.md:46-48), with agreement enforced."""
+    rows = bm.run_sweeps(k=20, repeats=1, max_n=20_000, n_jobs=16, sizes=(1_000, 5_000, 20_000),
+                         dims=(2, 8, 32, 128), n_dims=20_000)
     print(bm.HEADER)
     for r in rows:
         print(bm.format_row(r))
         assert r.max_rel_err <= bm.MAX_REL_ERR and r.rank_agreement == 1.0
         assert r.culof_s > 0 and r.points_per_s > 0
-    assert [r.n_features for r in rows] == [8, 8, 2, 8, 32, 128]
+    assert [(r.n_samples, r.n_features) for r in rows] == [
+        (1_000, 8), (5_000, 8), (20_000, 8), (20_000, 2), (20_000, 8), (20_000, 32), (20_000, 128)]
 
 
 @pytest.mark.parametrize("form", ["fp32", "bf16"])

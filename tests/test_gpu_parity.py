@@ -323,6 +323,23 @@ def test_overflow_guard_with_wide_outliers_at_scale():
         _assert_same_bits(gk[r], ok[0], f"row {r} kdist2")
 
 
+@pytest.mark.parametrize("d", [8, 32])
+def test_scratch_reuse_after_a_larger_call(d):
+    """A call reuses pool memory that a larger earlier call left behind
+    (positions, counts, candidate buffers); nothing of it may reach the
+    smaller call's result, which is bit-exact with the oracle."""
+    from _gpu import forward
+
+    print(f"scratch reuse after a larger call: d={d}", flush=True)
+    big = make_data(40_000, d, seed=31, n_outliers=400)
+    forward(big, 20)
+    small = make_data(2_000, d, seed=32, n_outliers=40)
+    got = forward(small, 20)
+    want = oracle.lof(small, 20, full=True)
+    np.testing.assert_array_equal(got["idx"].T, want["idx"])
+    _assert_same_bits(got["scores"], want["scores"], "scores")
+
+
 def test_small_scratch_budget_tiles_rows(monkeypatch):
     """CULOF_SCRATCH_BUDGET below the full run's collect buffers splits the
     query rows into tiles sized to the budget (the reference's tile_rows_for,
