@@ -375,6 +375,9 @@ void knn_rows(const Prepared& p, int n, int d, int k, int form, int row0, int ro
     // (no order, per-row lists in registers; the same bits).
     const bool order = path == Path::kMfma && ordering_enabled() && order_supported(p.stride) &&
                        !scratch.fixed();
+    // Without the candidate order the matrix cores run the list sweep, which
+    // holds k <= kListKMax; larger k then takes radix.
+    if (path == Path::kMfma && !order && !mfma_list_supported(d, k, form)) path = Path::kRadix;
     // Only the matrix-core path keeps per-row scratch (its collect buffers).
     auto tile_bytes = [&](int t) {
         if (path != Path::kMfma) return size_t{0};
@@ -475,7 +478,8 @@ size_t region_floor(const float* points, int n, int d, int k, bool normalize, in
         b += 2 * Scratch::footprint<int>(nn * static_cast<size_t>(k_sel)) + 2 * Scratch::footprint<int>(nn) +
              Scratch::footprint<int>(1);
     }
-    const Path path = choose_path(d, k_sel, screen, stride);
+    Path path = choose_path(d, k_sel, screen, stride);
+    if (path == Path::kMfma && !mfma_list_supported(d, k_sel, screen)) path = Path::kRadix;
     if (path != Path::kRadix) b += Scratch::footprint<int>(1) + Scratch::footprint<int>(nn);
     if (path == Path::kMfma) {
         b += Scratch::footprint<unsigned char>(

@@ -114,7 +114,15 @@ void sweep_query_knn(const float* qkeys, const float* qcentred, const float* qno
 // ---- mfma.hip: matrix-core screened sweep (FP32 or BF16 screen) ----
 // Row stride the form's screen expects (BF16 needs multiples of 16).
 int mfma_padded_dims(int n_dims, int form);
+// Largest k of the list kernels (the in-register list sweeps, the seeded list
+// pass); the collect pipeline (candidate order, seeds, collect, select) keeps
+// its lists in LDS and serves k up to kCollectKMax.
+constexpr int kListKMax = 64;
+constexpr int kCollectKMax = 256;
+// (n_dims, k, form) has a matrix-core collect pipeline instantiation.
 bool mfma_supported(int n_dims, int k, int form);
+// ... and a matrix-core list sweep (runs without the candidate order).
+bool mfma_list_supported(int n_dims, int k, int form);
 int mfma_slices(int n_points, int rows);
 // Device scratch the launch needs (screen operand copy, adjusted norms,
 // per-slice partial lists).

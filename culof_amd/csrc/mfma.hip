@@ -1216,17 +1216,26 @@ __device__ __forceinline__ uint64_t collected_word(const MfmaArgs& p, const floa
     return pack(kb, orig);
 }
 
-// The k smallest words of a row, in any order in sel[0, k) (k <= 64): written
-// in the reference's slot order by the calling wave (all 64 lanes).
+// The k smallest words of a row, in any order in sel[0, k) (k <= 64 KW):
+// written in the reference's slot order by the calling wave (all 64 lanes).
+template <int KW>
 __device__ void emit_selected(const MfmaArgs& p, const uint64_t* sel, int qi, int lane) {
-    // Called by a whole wave. Lane f holds word f; the k-th word is their
-    // maximum. Sorting the lanes by (group, index) -- group 0 for keys below
-    // the k-th, 1 for keys equal to it, 2 for lanes past k -- puts every word
-    // at its slot: below-k-th keys in ascending index, then the tied ones.
+    // Word e = 64 u + lane sits in register u of its lane; the k-th word is
+    // their maximum. Sorting the words by (group, index) -- group 0 for keys
+    // below the k-th, 1 for keys equal to it, 2 past k -- puts every word at
+    // its slot: below-k-th keys in ascending index, then the tied ones. The
+    // bitonic network runs over 64 KW words: strides below 64 between lanes
+    // (shuffles), the others between registers of one lane.
+    constexpr int N = 64 * KW;
     const int k = p.k;
-    const bool valid = lane < k;
-    const uint64_t me = valid ? sel[lane] : 0;
-    uint64_t kw = me;
+    uint64_t me[KW];
+    uint64_t kw = 0;
+#pragma unroll
+    for (int u = 0; u < KW; ++u) {
+        const int e = 64 * u + lane;
+        me[u] = e < k ? sel[e] : 0;
+        kw = me[u] > kw ? me[u] : kw;
+    }
 #pragma unroll
     for (int m = 1; m < 64; m <<= 1) {
         const uint32_t hi = static_cast<uint32_t>(__shfl_xor(static_cast<int>(kw >> 32), m, 64));
@@ -1235,38 +1244,80 @@ __device__ void emit_selected(const MfmaArgs& p, const uint64_t* sel, int qi, in
         kw = o > kw ? o : kw;
     }
     const uint32_t kth = static_cast<uint32_t>(kw >> 32);
-    uint32_t key = static_cast<uint32_t>(me >> 32);
-    const uint32_t grp = !valid ? 2u : (key < kth ? 0u : 1u);
-    uint64_t s = (static_cast<uint64_t>(grp) << 32) | static_cast<uint32_t>(me);
+    uint64_t s[KW];
+    uint32_t key[KW];
 #pragma unroll
-    for (int size = 2; size <= 64; size <<= 1) {
+    for (int u = 0; u < KW; ++u) {
+        key[u] = static_cast<uint32_t>(me[u] >> 32);
+        const uint32_t grp = 64 * u + lane >= k ? 2u : (key[u] < kth ? 0u : 1u);
+        s[u] = (static_cast<uint64_t>(grp) << 32) | static_cast<uint32_t>(me[u]);
+    }
+#pragma unroll
+    for (int size = 2; size <= N; size <<= 1) {
 #pragma unroll
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            const uint32_t ohi = static_cast<uint32_t>(__shfl_xor(static_cast<int>(s >> 32), stride, 64));
-            const uint32_t olo = static_cast<uint32_t>(__shfl_xor(static_cast<int>(s), stride, 64));
-            const uint32_t okey = static_cast<uint32_t>(__shfl_xor(static_cast<int>(key), stride, 64));
-            const uint64_t os = (static_cast<uint64_t>(ohi) << 32) | olo;
-            const bool up = (lane & size) == 0;
-            const bool lower = (lane & stride) == 0;
-            const bool swap = (lower == up) ? (os < s) : (os > s);
-            if (swap) {
-                s = os;
-                key = okey;
+            if (stride >= 64) {
+                const int su = stride / 64;
+#pragma unroll
+                for (int u = 0; u < KW; ++u) {
+                    if (u & su) continue;
+                    const int u2 = u | su;
+                    const bool up = ((64 * u + lane) & size) == 0;
+                    if (up ? s[u2] < s[u] : s[u2] > s[u]) {
+                        const uint64_t ts = s[u];
+                        s[u] = s[u2];
+                        s[u2] = ts;
+                        const uint32_t tk = key[u];
+                        key[u] = key[u2];
+                        key[u2] = tk;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < KW; ++u) {
+                    const uint32_t ohi = static_cast<uint32_t>(__shfl_xor(static_cast<int>(s[u] >> 32), stride, 64));
+                    const uint32_t olo = static_cast<uint32_t>(__shfl_xor(static_cast<int>(s[u]), stride, 64));
+                    const uint32_t okey = static_cast<uint32_t>(__shfl_xor(static_cast<int>(key[u]), stride, 64));
+                    const uint64_t os = (static_cast<uint64_t>(ohi) << 32) | olo;
+                    const bool up = ((64 * u + lane) & size) == 0;
+                    const bool lower = (lane & stride) == 0;
+                    const bool swap = (lower == up) ? (os < s[u]) : (os > s[u]);
+                    if (swap) {
+                        s[u] = os;
+                        key[u] = okey;
+                    }
+                }
             }
         }
     }
-    if (valid) {
-        const size_t ns = static_cast<size_t>(p.n);
-        p.nbr_idx[static_cast<size_t>(lane) * ns + qi] = static_cast<int>(static_cast<uint32_t>(s));
-        p.nbr_d2[static_cast<size_t>(lane) * ns + qi] = __uint_as_float(key);
-        if (lane == 0) p.kdist2[qi] = __uint_as_float(kth);
+    const size_t ns = static_cast<size_t>(p.n);
+#pragma unroll
+    for (int u = 0; u < KW; ++u) {
+        const int e = 64 * u + lane;
+        if (e < k) {
+            p.nbr_idx[static_cast<size_t>(e) * ns + qi] = static_cast<int>(static_cast<uint32_t>(s[u]));
+            p.nbr_d2[static_cast<size_t>(e) * ns + qi] = __uint_as_float(key[u]);
+        }
+    }
+    if (lane == 0) p.kdist2[qi] = __uint_as_float(kth);
+}
+
+// emit_selected for any k <= kCollectKMax.
+__device__ void emit_any(const MfmaArgs& p, const uint64_t* sel, int qi, int lane) {
+    if (p.k <= 64) {
+        emit_selected<1>(p, sel, qi, lane);
+    } else if (p.k <= 128) {
+        emit_selected<2>(p, sel, qi, lane);
+    } else {
+        emit_selected<4>(p, sel, qi, lane);
     }
 }
 
 // k smallest of the `total` words in s_words (WPL words per lane), emitted.
+// The selected words are compacted into s_words[0, k) (read whole by then)
+// for the emit.
 template <int WPL>
-__device__ void select_words(const MfmaArgs& p, const uint64_t* s_words, int total, int qi, int lane,
-                             uint64_t* s_sel) {
+__device__ void select_words(const MfmaArgs& p, uint64_t* s_words, int total, int qi, int lane) {
     uint64_t w[WPL];
 #pragma unroll
     for (int u = 0; u < WPL; ++u) {
@@ -1315,16 +1366,17 @@ __device__ void select_words(const MfmaArgs& p, const uint64_t* s_words, int tot
         if (lane == 0) queue_push(p.fb_rows, p.fb_count, p.fb_cap, qi);
         return;
     }
+    wave_lds_sync();  // every lane holds its words: the buffer takes the selection
     int off = 0;
 #pragma unroll
     for (int u = 0; u < WPL; ++u) {
         const bool m = w[u] <= ans;
         const uint64_t mask = __ballot(m);
-        if (m) s_sel[off + lanes_below(mask)] = w[u];
+        if (m) s_words[off + lanes_below(mask)] = w[u];
         off += __popcll(mask);
     }
     wave_lds_sync();
-    emit_selected(p, s_sel, qi, lane);
+    emit_any(p, s_words, qi, lane);
 }
 
 __device__ __forceinline__ int query_of(const MfmaArgs& p, int local) {
@@ -1341,7 +1393,6 @@ collect_select_kernel(MfmaArgs p) {
     __shared__ __attribute__((aligned(16))) float s_q[4][DP];
     constexpr int kWaveMax = wave_select_max<DP>();
     __shared__ uint64_t s_words[4][kWaveMax];
-    __shared__ uint64_t s_sel[4][64];
     __shared__ int s_pre[4][33];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -1407,13 +1458,13 @@ collect_select_kernel(MfmaArgs p) {
     }
     wave_lds_sync();
     if (total <= 128) {
-        select_words<2>(p, s_words[wave], total, qi, lane, s_sel[wave]);
+        select_words<2>(p, s_words[wave], total, qi, lane);
     } else if (total <= 256) {
-        select_words<4>(p, s_words[wave], total, qi, lane, s_sel[wave]);
+        select_words<4>(p, s_words[wave], total, qi, lane);
     } else if (kWaveMax <= 512 || total <= 512) {
-        select_words<8>(p, s_words[wave], total, qi, lane, s_sel[wave]);
+        select_words<8>(p, s_words[wave], total, qi, lane);
     } else if constexpr (kWaveMax > 512) {
-        select_words<16>(p, s_words[wave], total, qi, lane, s_sel[wave]);
+        select_words<16>(p, s_words[wave], total, qi, lane);
     }
 }
 
@@ -1534,7 +1585,7 @@ big_select_kernel(MfmaArgs p, const int* __restrict__ rows_list, const int* __re
             if (static_cast<uint32_t>(s_w[p.k - 1] >> 32) >= kInfBits) {
                 if (tid == 0) queue_push(p.fb_rows, p.fb_count, p.fb_cap, qi);
             } else {
-                emit_selected(p, s_w, qi, tid);
+                emit_any(p, s_w, qi, tid);
             }
         }
     }
@@ -1822,7 +1873,7 @@ void launch_sym(const MfmaArgs& a, const SymBuffers& sb, hipStream_t stream) {
 template <class Op, int DP>
 bool dispatch_k(const MfmaArgs* a, int k, hipStream_t stream) {
     if (a && a->collect) {
-        if (k > 64) return false;
+        if (k > kCollectKMax) return false;
         launch_collect<Op, DP>(*a, stream);
         return true;
     }
@@ -1892,6 +1943,12 @@ int mfma_padded_dims(int n_dims, int form) {
 }
 
 bool mfma_supported(int n_dims, int k, int form) {
+    const int dp = mfma_padded_dims(n_dims, form);
+    auto any = [](auto) { return true; };
+    return k <= kCollectKMax && (form == kFormBF16 ? for_dp<ScreenBF16>(dp, any) : for_dp<ScreenF32>(dp, any));
+}
+
+bool mfma_list_supported(int n_dims, int k, int form) {
     const int dp = mfma_padded_dims(n_dims, form);
     return form == kFormBF16 ? dispatch<ScreenBF16>(dp, k, nullptr, nullptr)
                              : dispatch<ScreenF32>(dp, k, nullptr, nullptr);
@@ -2006,7 +2063,7 @@ size_t mfma_scratch_bytes(int n_points, int dp, int form, int rows, int k, bool 
         lists = first + static_cast<size_t>(rows) * 2 * sizeof(int) + 4 * sizeof(int) +
                 second * kSecondCap * sizeof(uint32_t) + second * sizeof(int) +
                 static_cast<size_t>(std::min(rows, kListRows)) * (sizeof(int) +
-                    sizeof(uint64_t) * second_slices(n_points) * k) + 16 * 256;
+                    (k <= kListKMax ? sizeof(uint64_t) * second_slices(n_points) * k : 0)) + 16 * 256;
     } else if (s > 1) {
         lists = sizeof(uint64_t) * s * static_cast<size_t>(rows) * k;
     }
@@ -2110,7 +2167,7 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
         list.row0 = 0;
         list.rows = second.ov_cap;
         list.slices = second.slices;
-        list.partial = list.slices > 1 ? static_cast<uint64_t*>(take(
+        list.partial = list.slices > 1 && k <= kListKMax ? static_cast<uint64_t*>(take(
                            sizeof(uint64_t) * list.slices * static_cast<size_t>(list.rows) * k))
                                        : nullptr;
         list.cbuf = nullptr;
@@ -2160,7 +2217,7 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
             launch_sym<ScreenF32, decltype(D)::value>(args, sb, stream);
             return true;
         };
-        const bool ok = k <= 64 && (bf16 ? for_dp<ScreenBF16>(dp, go_bf16) : for_dp<ScreenF32>(dp, go_f32));
+        const bool ok = k <= kCollectKMax && (bf16 ? for_dp<ScreenBF16>(dp, go_bf16) : for_dp<ScreenF32>(dp, go_f32));
         if (!ok) throw std::invalid_argument("mfma_knn: unsupported (n_dims, k, form)");
         CULOF_CHECK(hipGetLastError());
     } else {
@@ -2168,7 +2225,14 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
     }
     if (collect) {
         run(second);
-        run(list);
+        if (k <= kListKMax) {
+            run(list);
+        } else {
+            // No list kernel holds this k: the rows queued for the list pass
+            // (heavy rows, second-pass overflow) take the exact radix select.
+            radix_knn(keyrows, stride, n_points, n_dims, k, 0, list.rows, list.qlist, list.row_count, nbr_idx,
+                      nbr_d2, kdist2, stream);
+        }
     }
 }
 
