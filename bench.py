@@ -40,6 +40,8 @@ torch.cuda.synchronize(). Prints one JSON line:
                d=128, k=50: points/s of each and whether the kNN sets, the
                k-distances and the scores are identical (and both forms on
                the headline workload)
+* shapes       n=200k forwards past the list kernels: k=128 (d=8, the headline
+               generator) and d=256 (k=20, low-rank clusters, both forms)
 """
 
 from __future__ import annotations
@@ -64,7 +66,7 @@ REFERENCE_PTS_S = 1.42e5      # reference on RTX A6000 at n=200k, d=8, k=20 (its
 KERNELS = ("center_partials", "center_apply", "order", "seed", "heavy_rows", "screen_prep",
            "mfma_collect", "scatter_pairs", "collect_select", "big_select",
            "second_collect",
-           "second_select", "mfma_sweep", "merge_slices", "sweep_knn", "radix_knn", "tie_break",
+           "second_select", "mfma_sweep", "merge_slices", "sweep_knn", "brute_rows", "radix_knn", "tie_break",
            "tie_collect", "tie_band", "lrd", "score")
 # Timing scope -> kernel name in the rocprofv3 counter summary.
 HBM_KERNELS = {"collect_select": "collect_select_kernel<8>", "scatter_pairs": "scatter_pairs_kernel"}
@@ -262,6 +264,31 @@ def forms_legs(seed: int, steps: int) -> dict:
         "note": "fit (forward on the fitted rows) and query kNN + scores in every call",
     }
     log(f"[bench] novelty {out['novelty']}")
+    del q, new
+
+    # Shapes past the list kernels: k = 128 on the headline generator, and
+    # rows of 256 features (16 low-rank clusters) in both screen forms.
+    shapes = []
+    for label, gen, dd, kk in (("k128", "clusters", 8, 128), ("d256", "lowrank", 256, 20)):
+        Xw = (workloads.clusters(n=200_000, d=dd, seed=seed) if gen == "clusters"
+              else workloads.lowrank(n=200_000, d=dd, seed=seed))
+        xw = torch.from_numpy(Xw).cuda()
+        res = {}
+        forms = ("fp32",) if dd < 32 else ("fp32", "bf16")
+        for form in forms:
+            t = _time_device(lambda f=form: res.__setitem__(f, forward_state(xw, kk, form=f)),
+                             max(2, steps // 2))
+            res[form + "_s"] = t
+        leg = {"shape": label, "generator": gen, "n": Xw.shape[0], "d": dd, "k": kk}
+        for form in forms:
+            leg[f"{form}_points_per_s"] = round(Xw.shape[0] / res[form + "_s"], 2)
+            leg[f"{form}_ms"] = round(res[form + "_s"] * 1e3, 3)
+        if len(forms) == 2:
+            leg["identical_knn"] = bool(torch.equal(res["fp32"].idx, res["bf16"].idx))
+        shapes.append(leg)
+        log(f"[bench] shape {leg}")
+        del xw, res
+    out["shapes"] = shapes
     return out
 
 

@@ -155,6 +155,17 @@ void radix_knn(const float* keyrows, int stride, int n_points, int n_dims, int k
 void radix_query_knn(const float* qrows, const float* keyrows, int stride, int n_points, int n_dims,
                      int k, int n_queries, const int* row_list, const int* row_count, int* nbr_idx,
                      float* nbr_d2, float* kdist2, hipStream_t stream);
+// Stored keys for listed rows: exact keys of row_list[0 .. min(*row_count,
+// cap)) against all n_points key rows, `batch` rows at a time into `keys`
+// (batch x n_points floats, brute_batch_rows), each batch then selected by
+// the stored radix select; outputs at the rows' original indices. The rows
+// many collected candidates would flood (heavy rows, overflowed buffers) take
+// it where no list kernel serves the shape.
+constexpr size_t kBruteKeyBytes = 512ull << 20;  // key buffer of one batch, at most
+int brute_batch_rows(int n_points, int cap);
+void brute_knn_rows(const float* keyrows, int stride, int n_points, int k, const int* row_list,
+                    const int* row_count, int cap, float* keys, int batch, int* nbr_idx, float* nbr_d2,
+                    float* kdist2, hipStream_t stream);
 // Stored mode: keys row-major (rows x n_cols); outputs with stride `rows`.
 void radix_select_rows(const float* keys, int rows, int n_cols, int k, int* nbr_idx,
                        float* nbr_d2, float* kdist2, hipStream_t stream);

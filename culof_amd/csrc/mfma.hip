@@ -62,6 +62,8 @@ constexpr uint64_t kSentinel = ~0ull;
 constexpr int kTilesPerBlock = 4;  // query tiles (waves) per block
 constexpr int kRing = 32;          // queued screen passes per lane
 constexpr float kU = 5.9604645e-8f;  // 2^-24
+constexpr int kWideMin = 256;    // padded widths from here on take the K-chunked collect pass
+constexpr int kWideChunk = 64;   // features per K chunk of that pass
 
 __device__ __forceinline__ uint64_t pack(uint32_t key, uint32_t idx) {
     return (static_cast<uint64_t>(key) << 32) | idx;
@@ -290,23 +292,30 @@ __device__ __forceinline__ float exact_key(const Q& q, const float* __restrict__
 }
 
 // exact_key with the query row in LDS (16-byte aligned), read as float4s.
+// Rows wider than 128 features go in unrolled pieces of 64 (the chain stays
+// in ascending feature order), so a key never holds a whole wide row in
+// registers.
 template <int DP>
 __device__ __forceinline__ float exact_key_q4(const float* q, const float* __restrict__ xs_row) {
-    const float4* q4 = reinterpret_cast<const float4*>(q);
-    const float4* c4 = reinterpret_cast<const float4*>(xs_row);
+    constexpr int kPiece = DP <= 128 ? DP : 64;
     float e = 0.0f;
+#pragma unroll 1
+    for (int f0 = 0; f0 < DP; f0 += kPiece) {
+        const float4* q4 = reinterpret_cast<const float4*>(q + f0);
+        const float4* c4 = reinterpret_cast<const float4*>(xs_row + f0);
 #pragma unroll
-    for (int v = 0; v < DP / 4; ++v) {
-        const float4 a = q4[v];
-        const float4 t = c4[v];
-        float df = a.x - t.x;
-        e = __builtin_fmaf(df, df, e);
-        df = a.y - t.y;
-        e = __builtin_fmaf(df, df, e);
-        df = a.z - t.z;
-        e = __builtin_fmaf(df, df, e);
-        df = a.w - t.w;
-        e = __builtin_fmaf(df, df, e);
+        for (int v = 0; v < kPiece / 4; ++v) {
+            const float4 a = q4[v];
+            const float4 t = c4[v];
+            float df = a.x - t.x;
+            e = __builtin_fmaf(df, df, e);
+            df = a.y - t.y;
+            e = __builtin_fmaf(df, df, e);
+            df = a.z - t.z;
+            e = __builtin_fmaf(df, df, e);
+            df = a.w - t.w;
+            e = __builtin_fmaf(df, df, e);
+        }
     }
     return e;
 }
@@ -374,6 +383,7 @@ struct MfmaArgs {
     int* fb_count;
     int fb_cap;          // entries fb_rows holds
     const int* heavy;    // symmetric pass: 1 for rows (by position) queued for the list pass up front
+    const int* qpos;     // wide rows: original index -> candidate position (null: identity)
 };
 
 // Streams candidate tiles [t_begin, t_end) in chunks of CT tiles through a
@@ -864,6 +874,186 @@ mfma_collect_kernel(MfmaArgs p) {
     if (valid && h == 0) p.ccount[slot] = cnt;
 }
 
+// ---- wide rows: the collect pass with the contraction in K chunks ----
+//
+// A query fragment of more than 128 features does not fit in registers beside
+// the candidate fragments, so for DP >= kWideMin the contraction runs in
+// chunks of kWideChunk features: each step stages one chunk of kWideTiles
+// candidate tiles in LDS (double-buffered through registers, one barrier per
+// step) and adds its products to those tiles' accumulators; the query chunk
+// comes from the operand rows themselves (position qpos[qi]), unscaled, so
+// the chain accumulates D = -m/2 + A x_q. Every intermediate of that chain is
+// exactly -1/2 times the one of C = m + A (-2 x_q) (scaling by a power of two
+// commutes with rounding), so acc = -2 D is the value the narrow pass would
+// screen, tested against the same threshold.
+constexpr int kWideTiles = 2;
+
+template <class Op, int DP>
+__global__ void __launch_bounds__(64 * kTilesPerBlock)
+mfma_wide_collect_kernel(MfmaArgs p) {
+    constexpr int KC = kWideChunk;
+    constexpr int NK = DP / KC;
+    constexpr int CT = kWideTiles;
+    static_assert(DP % KC == 0 && NK >= 2, "wide rows are whole chunks");
+    using Frag = typename Op::template Frag<KC>;
+    using Elem = typename Op::Elem;
+    constexpr int kRowF4 = KC * Op::kRowMul * static_cast<int>(sizeof(Elem)) / 16;  // float4s per chunk row
+    constexpr int kPitchF4 = kRowF4 + 1;    // LDS row pitch: the 32 rows of a read fall in different banks
+    constexpr int kSubF4 = 32 * kRowF4;     // one (tile, chunk) block in global memory
+    constexpr int kLdsTileF4 = 32 * kPitchF4;
+    constexpr int kStepF4 = CT * kSubF4;
+    constexpr int kLoads = (kStepF4 + 255) / 256;  // per thread
+    __shared__ float4 s_a[2][CT * kLdsTileF4];
+    __shared__ float4 s_m[2][CT * 8];
+
+    const int rows = p.row_count ? min(p.rows, *p.row_count) : p.rows;
+    if (static_cast<int>(blockIdx.x) * kTilesPerBlock * 32 >= rows) return;  // whole block
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int j = lane & 31;
+    const int h = lane >> 5;
+    const int local = (blockIdx.x * kTilesPerBlock + wave) * 32 + j;
+    const bool valid = local < rows;
+    const int qi = p.qlist ? p.qlist[valid ? local : 0] : p.row0 + (valid ? local : 0);
+
+    if (*p.guard) {
+        if (valid && h == 0 && blockIdx.y == 0) queue_push(p.fb_rows, p.fb_count, p.fb_cap, qi);
+        return;
+    }
+
+    const float4* ga = static_cast<const float4*>(p.xa);
+    const int qp = p.qpos ? p.qpos[qi] : qi;
+    // chunk 0 of the query's operand row; chunk kc is kSubF4 further on
+    const float4* qrow = ga + static_cast<size_t>(qp / 32) * NK * kSubF4 + (qp % 32) * kRowF4;
+    const float nq = p.norms[qi];
+    const float c = p.c;
+    const float t = p.tseed[qi];
+    const float Tn = valid ? (__builtin_fmaf(t, 1.0f + c, c * nq) - nq) + 4.0f * kU * (t + nq)
+                           : -__uint_as_float(kInfBits);
+    const size_t slot = static_cast<size_t>(valid ? local : 0) * p.slices + blockIdx.y;
+    uint32_t* buf = p.cbuf + slot * p.cap;
+    const int cap = p.cap;
+    int cnt = 0;  // the same in both lanes of a query
+
+    auto collect = [&](const floatx16& acc, int tile) {
+        float lo = acc[0];
+#pragma unroll
+        for (int r = 1; r < 16; ++r) lo = __builtin_fminf(lo, acc[r]);  // NaN rows drop out
+        if (__any(lo <= Tn)) {
+            uint32_t mask = 0;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) mask |= (acc[r] <= Tn ? 1u : 0u) << r;
+            const int mine = __popc(mask);
+            const int other = __shfl_xor(mine, 32, 64);
+            int pos = cnt + (h ? other : 0);
+            const uint32_t c0 = static_cast<uint32_t>(tile * 32 + 4 * h);
+            while (mask) {
+                const int r = __ffs(mask) - 1;
+                mask &= mask - 1;
+                if (pos < cap) buf[pos] = c0 + static_cast<uint32_t>(8 * (r / 4) + (r % 4));
+                ++pos;
+            }
+            cnt += mine + other;
+        }
+    };
+
+    const int n_tiles = (p.n + 31) / 32;
+    const int t_begin = static_cast<int>((static_cast<long long>(n_tiles) * blockIdx.y) / p.slices);
+    const int t_end = static_cast<int>((static_cast<long long>(n_tiles) * (blockIdx.y + 1)) / p.slices);
+    const int steps = (t_end - t_begin + CT - 1) / CT * NK;
+    const float4* gm = reinterpret_cast<const float4*>(p.madj);
+    const float nan = __uint_as_float(0x7FC00000u);
+    float4 ra[kLoads];
+    float4 rm = make_float4(nan, nan, nan, nan);
+    // Step s: tiles t_begin + CT (s / NK) + [0, CT), chunk s % NK; the tiles'
+    // norms travel with chunk 0 (threads [0, 8 CT), one float4 each).
+    auto fetch = [&](int st) {
+        const int g = st / NK;
+        const int kc = st - g * NK;
+        const int v0 = t_begin + g * CT;
+#pragma unroll
+        for (int u = 0; u < kLoads; ++u) {
+            const int idx = tid + 256 * u;
+            const int tt = idx / kSubF4;
+            const int vt = v0 + tt;
+            ra[u] = (idx < kStepF4 && vt < t_end)
+                        ? ga[(static_cast<size_t>(vt) * NK + kc) * kSubF4 + (idx - tt * kSubF4)]
+                        : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        if (kc == 0 && tid < CT * 8) {
+            const int vt = v0 + tid / 8;
+            rm = vt < t_end ? gm[static_cast<size_t>(vt) * 8 + tid % 8] : make_float4(nan, nan, nan, nan);
+        }
+    };
+    auto stage = [&](int st, int b) {
+#pragma unroll
+        for (int u = 0; u < kLoads; ++u) {
+            const int idx = tid + 256 * u;
+            if (idx < kStepF4) {
+                const int tt = idx / kSubF4;
+                const int r = idx - tt * kSubF4;
+                const int row = r / kRowF4;
+                s_a[b][tt * kLdsTileF4 + row * kPitchF4 + (r - row * kRowF4)] = ra[u];
+            }
+        }
+        if (st % NK == 0 && tid < CT * 8) s_m[b][tid] = rm;
+    };
+
+    floatx16 acc[CT];
+    if (steps > 0) {
+        fetch(0);
+        stage(0, 0);
+    }
+    __syncthreads();
+    for (int st = 0; st < steps; ++st) {
+        const int b = st & 1;
+        const int g = st / NK;
+        const int kc = st - g * NK;
+        if (st + 1 < steps) fetch(st + 1);  // next step in flight
+        if (kc == 0) {
+#pragma unroll
+            for (int tt = 0; tt < CT; ++tt) {
+#pragma unroll
+                for (int bb = 0; bb < 4; ++bb) {
+                    const float4 m4 = s_m[b][tt * 8 + 2 * bb + h];
+                    acc[tt][4 * bb + 0] = -0.5f * m4.x;
+                    acc[tt][4 * bb + 1] = -0.5f * m4.y;
+                    acc[tt][4 * bb + 2] = -0.5f * m4.z;
+                    acc[tt][4 * bb + 3] = -0.5f * m4.w;
+                }
+            }
+        }
+        Frag bq;
+        Op::template load_a<KC>(reinterpret_cast<const Elem*>(qrow + static_cast<size_t>(kc) * kSubF4) +
+                                    Op::half_offset(KC, h),
+                                bq);
+#pragma unroll
+        for (int tt = 0; tt < CT; ++tt) {
+            Frag fa;
+            Op::template load_a<KC>(reinterpret_cast<const Elem*>(&s_a[b][tt * kLdsTileF4 + j * kPitchF4]) +
+                                        Op::half_offset(KC, h),
+                                    fa);
+            acc[tt] = Op::template run<KC>(fa, bq, acc[tt]);
+        }
+        if (kc == NK - 1) {
+#pragma unroll
+            for (int tt = 0; tt < CT; ++tt) {
+                const int v = t_begin + g * CT + tt;
+                if (v < t_end) {
+                    floatx16 a2;
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) a2[r] = -2.0f * acc[tt][r];
+                    collect(a2, v);
+                }
+            }
+        }
+        if (st + 1 < steps) stage(st + 1, b ^ 1);
+        __syncthreads();
+    }
+    if (valid && h == 0) p.ccount[slot] = cnt;
+}
+
 // ---- symmetric collect: a full run, queries and candidates in one order ----
 //
 // One screen value serves both rows of a pair. With acc = m_c - 2 x_q . x_c
@@ -1216,10 +1406,122 @@ __device__ __forceinline__ uint64_t collected_word(const MfmaArgs& p, const floa
     return pack(kb, orig);
 }
 
-// The k smallest words of a row, in any order in sel[0, k) (k <= 64 KW):
-// written in the reference's slot order by the calling wave (all 64 lanes).
-template <int KW>
+// The k smallest words of a row, in any order in sel[0, k) (k <= 64): written
+// in the reference's slot order by the calling wave (all 64 lanes).
 __device__ void emit_selected(const MfmaArgs& p, const uint64_t* sel, int qi, int lane) {
+    // Called by a whole wave. Lane f holds word f; the k-th word is their
+    // maximum. Sorting the lanes by (group, index) -- group 0 for keys below
+    // the k-th, 1 for keys equal to it, 2 for lanes past k -- puts every word
+    // at its slot: below-k-th keys in ascending index, then the tied ones.
+    const int k = p.k;
+    const bool valid = lane < k;
+    const uint64_t me = valid ? sel[lane] : 0;
+    uint64_t kw = me;
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        const uint32_t hi = static_cast<uint32_t>(__shfl_xor(static_cast<int>(kw >> 32), m, 64));
+        const uint32_t lo = static_cast<uint32_t>(__shfl_xor(static_cast<int>(kw), m, 64));
+        const uint64_t o = (static_cast<uint64_t>(hi) << 32) | lo;
+        kw = o > kw ? o : kw;
+    }
+    const uint32_t kth = static_cast<uint32_t>(kw >> 32);
+    uint32_t key = static_cast<uint32_t>(me >> 32);
+    const uint32_t grp = !valid ? 2u : (key < kth ? 0u : 1u);
+    uint64_t s = (static_cast<uint64_t>(grp) << 32) | static_cast<uint32_t>(me);
+#pragma unroll
+    for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const uint32_t ohi = static_cast<uint32_t>(__shfl_xor(static_cast<int>(s >> 32), stride, 64));
+            const uint32_t olo = static_cast<uint32_t>(__shfl_xor(static_cast<int>(s), stride, 64));
+            const uint32_t okey = static_cast<uint32_t>(__shfl_xor(static_cast<int>(key), stride, 64));
+            const uint64_t os = (static_cast<uint64_t>(ohi) << 32) | olo;
+            const bool up = (lane & size) == 0;
+            const bool lower = (lane & stride) == 0;
+            const bool swap = (lower == up) ? (os < s) : (os > s);
+            if (swap) {
+                s = os;
+                key = okey;
+            }
+        }
+    }
+    if (valid) {
+        const size_t ns = static_cast<size_t>(p.n);
+        p.nbr_idx[static_cast<size_t>(lane) * ns + qi] = static_cast<int>(static_cast<uint32_t>(s));
+        p.nbr_d2[static_cast<size_t>(lane) * ns + qi] = __uint_as_float(key);
+        if (lane == 0) p.kdist2[qi] = __uint_as_float(kth);
+    }
+}
+
+// k smallest of the `total` words in s_words (WPL words per lane), emitted.
+template <int WPL>
+__device__ void select_words(const MfmaArgs& p, const uint64_t* s_words, int total, int qi, int lane,
+                             uint64_t* s_sel) {
+    uint64_t w[WPL];
+#pragma unroll
+    for (int u = 0; u < WPL; ++u) {
+        const int e = lane + 64 * u;
+        w[u] = e < total ? s_words[e] : kSentinel;
+    }
+    // The k-th smallest word: the k-th key by a bitwise search on the key
+    // bits, then, only when that key is tied, the right index among the tied.
+    // (Keys are at most +inf, bit 31 clear; at least k of them are real.)
+    uint32_t kh = 0;
+    for (int b = 30; b >= 0; --b) {
+        const uint32_t probe = kh | (1u << b);
+        int below = 0;
+#pragma unroll
+        for (int u = 0; u < WPL; ++u) below += __popcll(__ballot(static_cast<uint32_t>(w[u] >> 32) < probe));
+        if (below < p.k) kh = probe;
+    }
+    int below = 0, tied = 0;
+#pragma unroll
+    for (int u = 0; u < WPL; ++u) {
+        const uint32_t hi = static_cast<uint32_t>(w[u] >> 32);
+        below += __popcll(__ballot(hi < kh));
+        tied += __popcll(__ballot(hi == kh));
+    }
+    const int need = p.k - below;  // 1 <= need <= tied
+    uint32_t lo = 0;
+    if (need < tied) {
+        for (int b = 31; b >= 0; --b) {
+            const uint32_t probe = lo | (1u << b);
+            int cnt = 0;
+#pragma unroll
+            for (int u = 0; u < WPL; ++u)
+                cnt += __popcll(__ballot(static_cast<uint32_t>(w[u] >> 32) == kh && static_cast<uint32_t>(w[u]) < probe));
+            if (cnt < need) lo = probe;
+        }
+    } else {
+        // every tied word is taken: the k-th is the largest of them
+#pragma unroll
+        for (int u = 0; u < WPL; ++u)
+            if (static_cast<uint32_t>(w[u] >> 32) == kh) lo = max(lo, static_cast<uint32_t>(w[u]));
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1) lo = max(lo, static_cast<uint32_t>(__shfl_xor(static_cast<int>(lo), m, 64)));
+    }
+    const uint64_t ans = (static_cast<uint64_t>(kh) << 32) | lo;
+    if (kh >= kInfBits) {
+        if (lane == 0) queue_push(p.fb_rows, p.fb_count, p.fb_cap, qi);
+        return;
+    }
+    int off = 0;
+#pragma unroll
+    for (int u = 0; u < WPL; ++u) {
+        const bool m = w[u] <= ans;
+        const uint64_t mask = __ballot(m);
+        if (m) s_sel[off + lanes_below(mask)] = w[u];
+        off += __popcll(mask);
+    }
+    wave_lds_sync();
+    emit_selected(p, s_sel, qi, lane);
+}
+
+// emit_selected for 64 < k <= 64 KW: the same slot order, the bitonic network
+// over 64 KW words (strides below 64 between lanes by shuffles, the others
+// between registers of one lane).
+template <int KW>
+__device__ __forceinline__ void emit_selected_wide(const MfmaArgs& p, const uint64_t* sel, int qi, int lane) {
     // Word e = 64 u + lane sits in register u of its lane; the k-th word is
     // their maximum. Sorting the words by (group, index) -- group 0 for keys
     // below the k-th, 1 for keys equal to it, 2 past k -- puts every word at
@@ -1302,22 +1604,20 @@ __device__ void emit_selected(const MfmaArgs& p, const uint64_t* sel, int qi, in
     if (lane == 0) p.kdist2[qi] = __uint_as_float(kth);
 }
 
-// emit_selected for any k <= kCollectKMax.
-__device__ void emit_any(const MfmaArgs& p, const uint64_t* sel, int qi, int lane) {
-    if (p.k <= 64) {
-        emit_selected<1>(p, sel, qi, lane);
-    } else if (p.k <= 128) {
-        emit_selected<2>(p, sel, qi, lane);
+// The emit for 64 < k <= kCollectKMax (the selects' WIDE instantiations).
+__device__ __forceinline__ void emit_wide(const MfmaArgs& p, const uint64_t* sel, int qi, int lane) {
+    if (p.k <= 128) {
+        emit_selected_wide<2>(p, sel, qi, lane);
     } else {
-        emit_selected<4>(p, sel, qi, lane);
+        emit_selected_wide<4>(p, sel, qi, lane);
     }
 }
 
-// k smallest of the `total` words in s_words (WPL words per lane), emitted.
-// The selected words are compacted into s_words[0, k) (read whole by then)
-// for the emit.
+// select_words for k > 64: the selected words are compacted into
+// s_words[0, k) (read whole by then) for the wide emit.
 template <int WPL>
-__device__ void select_words(const MfmaArgs& p, uint64_t* s_words, int total, int qi, int lane) {
+__device__ __forceinline__ void select_words_wide(const MfmaArgs& p, uint64_t* s_words, int total, int qi,
+                                                  int lane) {
     uint64_t w[WPL];
 #pragma unroll
     for (int u = 0; u < WPL; ++u) {
@@ -1376,7 +1676,20 @@ __device__ void select_words(const MfmaArgs& p, uint64_t* s_words, int total, in
         off += __popcll(mask);
     }
     wave_lds_sync();
-    emit_any(p, s_words, qi, lane);
+    emit_wide(p, s_words, qi, lane);
+}
+
+// The wave select for k > 64 (the WIDE instantiation of the select kernel).
+__device__ __forceinline__ void wide_select(const MfmaArgs& p, uint64_t* s_words, int total, int qi, int lane) {
+    if (total <= 128) {
+        select_words_wide<2>(p, s_words, total, qi, lane);
+    } else if (total <= 256) {
+        select_words_wide<4>(p, s_words, total, qi, lane);
+    } else if (total <= 512) {
+        select_words_wide<8>(p, s_words, total, qi, lane);
+    } else {
+        select_words_wide<16>(p, s_words, total, qi, lane);
+    }
 }
 
 __device__ __forceinline__ int query_of(const MfmaArgs& p, int local) {
@@ -1387,12 +1700,15 @@ __device__ __forceinline__ int query_of(const MfmaArgs& p, int local) {
 // are keyed and selected here; larger ones are queued for the block select
 // (big_rows) or, when a slice buffer overflowed, for a second collect pass
 // with larger buffers (ov_rows).
-template <int DP>
+// WIDE: the instantiation for 64 < k <= kCollectKMax (the k <= 64 one keeps
+// its registers and occupancy free of the wider emit).
+template <int DP, bool WIDE>
 __global__ void __launch_bounds__(256)
 collect_select_kernel(MfmaArgs p) {
     __shared__ __attribute__((aligned(16))) float s_q[4][DP];
     constexpr int kWaveMax = wave_select_max<DP>();
     __shared__ uint64_t s_words[4][kWaveMax];
+    __shared__ uint64_t s_sel[4][64];
     __shared__ int s_pre[4][33];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -1457,14 +1773,16 @@ collect_select_kernel(MfmaArgs p) {
         s_words[wave][e] = collected_word<DP>(p, s_q[wave], pos, qi);
     }
     wave_lds_sync();
-    if (total <= 128) {
-        select_words<2>(p, s_words[wave], total, qi, lane);
+    if constexpr (WIDE) {
+        wide_select(p, s_words[wave], total, qi, lane);
+    } else if (total <= 128) {
+        select_words<2>(p, s_words[wave], total, qi, lane, s_sel[wave]);
     } else if (total <= 256) {
-        select_words<4>(p, s_words[wave], total, qi, lane);
+        select_words<4>(p, s_words[wave], total, qi, lane, s_sel[wave]);
     } else if (kWaveMax <= 512 || total <= 512) {
-        select_words<8>(p, s_words[wave], total, qi, lane);
+        select_words<8>(p, s_words[wave], total, qi, lane, s_sel[wave]);
     } else if constexpr (kWaveMax > 512) {
-        select_words<16>(p, s_words[wave], total, qi, lane);
+        select_words<16>(p, s_words[wave], total, qi, lane, s_sel[wave]);
     }
 }
 
@@ -1480,7 +1798,7 @@ collect_select_kernel(MfmaArgs p) {
 constexpr int kBigThreads = 512;
 constexpr int kBigMinRoom = 512;
 
-template <int DP>
+template <int DP, bool WIDE>
 __global__ void __launch_bounds__(kBigThreads)
 big_select_kernel(MfmaArgs p, const int* __restrict__ rows_list, const int* __restrict__ rows_count) {
     __shared__ __attribute__((aligned(16))) float s_q[DP];
@@ -1585,7 +1903,11 @@ big_select_kernel(MfmaArgs p, const int* __restrict__ rows_list, const int* __re
             if (static_cast<uint32_t>(s_w[p.k - 1] >> 32) >= kInfBits) {
                 if (tid == 0) queue_push(p.fb_rows, p.fb_count, p.fb_cap, qi);
             } else {
-                emit_any(p, s_w, qi, tid);
+                if constexpr (WIDE) {
+                    emit_wide(p, s_w, qi, tid);
+                } else {
+                    emit_selected(p, s_w, qi, tid);
+                }
             }
         }
     }
@@ -1658,6 +1980,7 @@ band_select_kernel(MfmaArgs p, const double* __restrict__ bound_of, int* __restr
             const float e32 = exact_key_q4<DP>(s_q, row);
             if (!(static_cast<double>(e32) <= bound)) continue;  // NaN stays out
             double e64 = 0.0;
+#pragma unroll 16
             for (int t = 0; t < DP; ++t) {  // zero padding adds exact zeros
                 const double df = __dsub_rn(static_cast<double>(s_q[t]), static_cast<double>(row[t]));
                 e64 = __dadd_rn(e64, __dmul_rn(df, df));
@@ -1738,6 +2061,22 @@ merge_slices_kernel(const uint64_t* __restrict__ partial, int slices, int rows, 
 }
 
 // ---- operand preparation ----
+//
+// Rows of DP <= 128 features are stored whole in the form layout. Wider rows
+// (the K-chunked collect pass) are stored in chunks of kWideChunk features:
+// block (tile, chunk) holds the 32 rows of a candidate tile, each as the form
+// layout of that chunk's features, so one chunk of a tile is contiguous.
+// Element (row i, feature f) of a padded width dp in the operand layout, in
+// elements (floats for FP32; bf16 values for the split form, hi then lo).
+__device__ __forceinline__ size_t operand_at(size_t i, int f, int dp, bool bf16, bool lo) {
+    const bool wide = dp >= kWideMin;
+    const int w = wide ? kWideChunk : dp;  // features per stored (sub)row
+    const int kc = wide ? f / kWideChunk : 0;
+    const int fl = f - kc * w;
+    const size_t sub = wide ? ((i / 32) * (dp / kWideChunk) + kc) * 32 + i % 32 : i;
+    if (bf16) return sub * 2 * w + (lo ? w : 0) + fl;
+    return sub * w + ((fl & 1) ? w / 2 + fl / 2 : fl / 2);
+}
 
 // Screen operands for rows [0, padded_rows): rows past n are zero.
 // Screen operands for candidate positions [0, padded_rows) (position p holds
@@ -1748,15 +2087,14 @@ screen_rows_f32_kernel(const float* __restrict__ xc, const int* __restrict__ can
     const size_t cell = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (cell >= static_cast<size_t>(padded_rows) * dp) return;
     const size_t i = cell / dp;
-    const int pos = static_cast<int>(cell % dp);
-    const int half = dp / 2;
-    const int src = pos < half ? 2 * pos : 2 * (pos - half) + 1;
+    const int f = static_cast<int>(cell % dp);
+    const size_t at = operand_at(i, f, dp, false, false);
     if (i >= static_cast<size_t>(n)) {
-        xa[cell] = 0.0f;
+        xa[at] = 0.0f;
         return;
     }
     const size_t row = cand ? static_cast<size_t>(cand[i]) : i;
-    xa[cell] = xc[row * dp + src];
+    xa[at] = xc[row * dp + f];
 }
 
 // Split BF16 operands: row p = [bf16(x) for each feature, bf16(x - bf16(x)) ...].
@@ -1766,18 +2104,19 @@ screen_rows_bf16_kernel(const float* __restrict__ xc, const int* __restrict__ ca
     const size_t cell = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (cell >= padded_cells) return;
     const size_t i = cell / dp;
-    const size_t f = cell - i * dp;
-    __bf16* row = xb + i * 2 * dp;
+    const int f = static_cast<int>(cell - i * dp);
+    const size_t at_hi = operand_at(i, f, dp, true, false);
+    const size_t at_lo = operand_at(i, f, dp, true, true);
     if (i >= static_cast<size_t>(n)) {
-        row[f] = static_cast<__bf16>(0.0f);
-        row[dp + f] = static_cast<__bf16>(0.0f);
+        xb[at_hi] = static_cast<__bf16>(0.0f);
+        xb[at_lo] = static_cast<__bf16>(0.0f);
         return;
     }
     const size_t src = cand ? static_cast<size_t>(cand[i]) : i;
     const float v = xc[src * dp + f];
     const __bf16 hi = static_cast<__bf16>(v);
-    row[f] = hi;
-    row[dp + f] = static_cast<__bf16>(v - static_cast<float>(hi));
+    xb[at_hi] = hi;
+    xb[at_lo] = static_cast<__bf16>(v - static_cast<float>(hi));
 }
 
 // |x|^2 (1 - c) per candidate position in [0, padded_rows); NaN past n,
@@ -1813,20 +2152,37 @@ void launch_collect(const MfmaArgs& a, hipStream_t stream) {
     const unsigned big_grid = static_cast<unsigned>(std::min(a.rows, 1024));
     {
         ProfileScope prof(a.big_rows ? "mfma_collect" : "second_collect", stream);
-        mfma_collect_kernel<Op, DP><<<grid, 64 * kTilesPerBlock, 0, stream>>>(a);
+        if constexpr (DP >= kWideMin) {
+            mfma_wide_collect_kernel<Op, DP><<<grid, 64 * kTilesPerBlock, 0, stream>>>(a);
+        } else {
+            mfma_collect_kernel<Op, DP><<<grid, 64 * kTilesPerBlock, 0, stream>>>(a);
+        }
     }
+    const bool wide = a.k > kListKMax;
     if (a.big_rows) {
         // First pass: a wave per row, large sets to the block select.
         {
             ProfileScope prof("collect_select", stream);
-            collect_select_kernel<DP><<<(a.rows + 3) / 4, 256, 0, stream>>>(a);
+            if (wide) {
+                collect_select_kernel<DP, true><<<(a.rows + 3) / 4, 256, 0, stream>>>(a);
+            } else {
+                collect_select_kernel<DP, false><<<(a.rows + 3) / 4, 256, 0, stream>>>(a);
+            }
         }
         ProfileScope prof("big_select", stream);
-        big_select_kernel<DP><<<big_grid, kBigThreads, 0, stream>>>(a, a.big_rows, a.big_count);
+        if (wide) {
+            big_select_kernel<DP, true><<<big_grid, kBigThreads, 0, stream>>>(a, a.big_rows, a.big_count);
+        } else {
+            big_select_kernel<DP, false><<<big_grid, kBigThreads, 0, stream>>>(a, a.big_rows, a.big_count);
+        }
     } else {
         // Second pass over overflowed rows: all large.
         ProfileScope prof("second_select", stream);
-        big_select_kernel<DP><<<big_grid, kBigThreads, 0, stream>>>(a, nullptr, a.row_count);
+        if (wide) {
+            big_select_kernel<DP, true><<<big_grid, kBigThreads, 0, stream>>>(a, nullptr, a.row_count);
+        } else {
+            big_select_kernel<DP, false><<<big_grid, kBigThreads, 0, stream>>>(a, nullptr, a.row_count);
+        }
     }
 }
 
@@ -1862,12 +2218,21 @@ void launch_sym(const MfmaArgs& a, const SymBuffers& sb, hipStream_t stream) {
                                                       a.guard, a.cbuf, a.ccount);
         mark_overflow_kernel<<<(a.n + 255) / 256, 256, 0, stream>>>(sb.col_over, a.n, sb.slices, a.cap, a.ccount);
     }
+    const bool wide = a.k > kListKMax;
     {
         ProfileScope prof("collect_select", stream);
-        collect_select_kernel<DP><<<(a.rows + 3) / 4, 256, 0, stream>>>(a);
+        if (wide) {
+            collect_select_kernel<DP, true><<<(a.rows + 3) / 4, 256, 0, stream>>>(a);
+        } else {
+            collect_select_kernel<DP, false><<<(a.rows + 3) / 4, 256, 0, stream>>>(a);
+        }
     }
     ProfileScope prof("big_select", stream);
-    big_select_kernel<DP><<<std::min(a.rows, 1024), kBigThreads, 0, stream>>>(a, a.big_rows, a.big_count);
+    if (wide) {
+        big_select_kernel<DP, true><<<std::min(a.rows, 1024), kBigThreads, 0, stream>>>(a, a.big_rows, a.big_count);
+    } else {
+        big_select_kernel<DP, false><<<std::min(a.rows, 1024), kBigThreads, 0, stream>>>(a, a.big_rows, a.big_count);
+    }
 }
 
 template <class Op, int DP>
@@ -1877,23 +2242,27 @@ bool dispatch_k(const MfmaArgs* a, int k, hipStream_t stream) {
         launch_collect<Op, DP>(*a, stream);
         return true;
     }
-    if (k <= 16) {
-        if (a) launch_mfma<Op, DP, 16>(*a, stream);
-        return true;
+    if constexpr (DP >= kWideMin) {
+        return false;  // no list sweep for wide rows
+    } else {
+        if (k <= 16) {
+            if (a) launch_mfma<Op, DP, 16>(*a, stream);
+            return true;
+        }
+        if (k <= 32) {
+            if (a) launch_mfma<Op, DP, 32>(*a, stream);
+            return true;
+        }
+        if (k <= 48) {  // e.g. k + 1 = 33 of the FP64 tie-break at k = 32: two waves per SIMD
+            if (a) launch_mfma<Op, DP, 48>(*a, stream);
+            return true;
+        }
+        if (k <= 64) {
+            if (a) launch_mfma<Op, DP, 64>(*a, stream);
+            return true;
+        }
+        return false;
     }
-    if (k <= 32) {
-        if (a) launch_mfma<Op, DP, 32>(*a, stream);
-        return true;
-    }
-    if (k <= 48) {  // e.g. k + 1 = 33 of the FP64 tie-break at k = 32: two waves per SIMD
-        if (a) launch_mfma<Op, DP, 48>(*a, stream);
-        return true;
-    }
-    if (k <= 64) {
-        if (a) launch_mfma<Op, DP, 64>(*a, stream);
-        return true;
-    }
-    return false;
 }
 
 // Calls f(std::integral_constant<int, DP>) for the padded widths Op is built for.
@@ -1912,6 +2281,8 @@ bool for_dp(int dp, F&& f) {
         case 32: return f(integral_constant<int, 32>{});
         case 64: return f(integral_constant<int, 64>{});
         case 128: return f(integral_constant<int, 128>{});
+        case 256: return f(integral_constant<int, 256>{});
+        case 512: return f(integral_constant<int, 512>{});
         default: return false;
     }
 }
@@ -1929,7 +2300,11 @@ void launch_band(const MfmaArgs& a, const double* bound, int* brute_rows, int* b
     const dim3 grid((qtiles + kTilesPerBlock - 1) / kTilesPerBlock, a.slices);
     {
         ProfileScope prof("tie_collect", stream);
-        mfma_collect_kernel<Op, DP><<<grid, 64 * kTilesPerBlock, 0, stream>>>(a);
+        if constexpr (DP >= kWideMin) {
+            mfma_wide_collect_kernel<Op, DP><<<grid, 64 * kTilesPerBlock, 0, stream>>>(a);
+        } else {
+            mfma_collect_kernel<Op, DP><<<grid, 64 * kTilesPerBlock, 0, stream>>>(a);
+        }
     }
     ProfileScope prof("tie_band", stream);
     band_select_kernel<DP><<<std::min(a.rows, 1024), 256, 0, stream>>>(a, bound, brute_rows, brute_count);
@@ -1965,11 +2340,15 @@ int mfma_slices(int n_points, int rows) {
     return static_cast<int>(std::max<long long>(s, 1));
 }
 
-// Per-(query, slice, half) buffer capacity of collect mode.
-// (Each half of a slice may hold up to half of what the select keeps.)
-int collect_cap(int) { return kSelectMax / 2; }
+// First-pass buffer capacity per (row, slice): kSelectMax entries up to
+// k = 64; beyond, 16 k (at most 4096), since a seed's candidate set grows with
+// k (it is the k-th key among a fixed window) and a full slice sends the row
+// to the slower passes.
+int first_cap(int k) {
+    return k <= kListKMax ? kSelectMax : std::min(4096, (16 * k + 255) / 256 * 256);
+}
 
-// Collect buffers: kSelectMax positions per (query, slice) in the first pass;
+// Collect buffers: first_cap(k) positions per (query, slice) in the first pass;
 // rows whose slice buffer overflows (at most kSecondRows of them) are
 // collected again with kSecondCap per slice, then go to the block select.
 // The second pass serves few rows, so it splits the candidates into many
@@ -2018,10 +2397,9 @@ int second_slices(int n_points) {
 }
 
 // Symmetric first pass (a full run with the candidate order): 2S slices of
-// kSymCap entries per row (S row slices, S column slices); a row with a full
+// first_cap(k) entries per row (S row slices, S column slices); a row with a full
 // slice goes to the second pass.
 // CULOF_SYM=0 turns it off (the same results; for measurement).
-constexpr int kSymCap = 1024;
 constexpr int kSymMinRows = 32768;
 bool sym_enabled() {
     const char* env = std::getenv("CULOF_SYM");
@@ -2053,17 +2431,21 @@ size_t mfma_scratch_bytes(int n_points, int dp, int form, int rows, int k, bool 
         const size_t slots = static_cast<size_t>(rows) * s;
         const size_t second = static_cast<size_t>(std::min(rows, kSecondRows)) * second_slices(n_points);
         const size_t ss = static_cast<size_t>(sym_slices(n_points, sym_qt(dp, form)));
-        const size_t sym = static_cast<size_t>(rows) * (2 * ss * (kSymCap + 1) + 2) * sizeof(uint32_t) +
+        const size_t sym = static_cast<size_t>(rows) * (2 * ss * (first_cap(k) + 1) + 2) * sizeof(uint32_t) +
                            padded * sizeof(float) + padded / 32 * sizeof(float2) +
                            static_cast<size_t>(sym_chunks(n_points)) * (kChunkWords * sizeof(uint64_t) + 2 * sizeof(int)) +
                            2048;
         // (the symmetric pass serves full runs only)
         const bool sym_run = rows == n_points && n_points >= kSymMinRows;
-        const size_t first = std::max(slots * (kSelectMax + 1) * sizeof(uint32_t), sym_run ? sym : 0) + 256;
+        const size_t first = std::max(slots * (first_cap(k) + 1) * sizeof(uint32_t), sym_run ? sym : 0) + 256;
         lists = first + static_cast<size_t>(rows) * 2 * sizeof(int) + 4 * sizeof(int) +
                 second * kSecondCap * sizeof(uint32_t) + second * sizeof(int) +
                 static_cast<size_t>(std::min(rows, kListRows)) * (sizeof(int) +
-                    (k <= kListKMax ? sizeof(uint64_t) * second_slices(n_points) * k : 0)) + 16 * 256;
+                    (k <= kListKMax && dp < kWideMin ? sizeof(uint64_t) * second_slices(n_points) * k : 0)) + 16 * 256;
+        if (k > kListKMax || dp >= kWideMin) {  // the list rows' stored keys
+            lists += static_cast<size_t>(brute_batch_rows(n_points, std::min(rows, kListRows))) * n_points *
+                     sizeof(float) + 256;
+        }
     } else if (s > 1) {
         lists = sizeof(uint64_t) * s * static_cast<size_t>(rows) * k;
     }
@@ -2098,23 +2480,26 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
                   qlist, tseed, madj, norms, guard, n_points, k, c, row0, rows,
                   slices, nullptr, collect ? 1 : 0, nullptr, nullptr, nullptr, kSelectMax,
                   nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr, 0, nbr_idx, nbr_d2,
-                  kdist2, fallback_rows, fallback_count, fallback_cap};
+                  kdist2, fallback_rows, fallback_count, fallback_cap, nullptr, ord ? ord->inv : nullptr};
     MfmaArgs second = args;
     MfmaArgs list = args;
+    float* brute_keys = nullptr;  // stored keys of the list rows where no list kernel serves
+    int brute_batch = 0;
     // A full run in candidate order computes each pair of tiles once.
     // (Below kSymMinRows the extra launches of the symmetric pass cost more
     // than the halved matrix work saves.)
+    // (Wide rows take the K-chunked pass, which has no symmetric form.)
     const bool sym = collect && ord && qlist == ord->order && row0 == 0 && rows == n_points &&
-                     n_points >= kSymMinRows && sym_enabled();
+                     n_points >= kSymMinRows && dp < kWideMin && sym_enabled();
     SymBuffers sb{};
     int* heavy = nullptr;
     if (collect) {
         if (sym) {
             sb.slices = sym_slices(n_points, sym_qt(dp, form));
             args.slices = 2 * sb.slices;
-            args.cap = kSymCap;
+            args.cap = first_cap(k);
             const size_t slots = static_cast<size_t>(rows) * args.slices;
-            args.cbuf = static_cast<uint32_t*>(take(slots * kSymCap * sizeof(uint32_t)));
+            args.cbuf = static_cast<uint32_t*>(take(slots * args.cap * sizeof(uint32_t)));
             args.ccount = static_cast<int*>(take(slots * sizeof(int)));
             heavy = static_cast<int*>(take(static_cast<size_t>(rows) * sizeof(int)));
             args.heavy = heavy;
@@ -2132,7 +2517,8 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
             zero_fill(sb.col_over, static_cast<size_t>(rows) * sizeof(int), stream);
         } else {
             const size_t slots = static_cast<size_t>(rows) * slices;
-            args.cbuf = static_cast<uint32_t*>(take(slots * kSelectMax * sizeof(uint32_t)));
+            args.cap = first_cap(k);
+            args.cbuf = static_cast<uint32_t*>(take(slots * args.cap * sizeof(uint32_t)));
             args.ccount = static_cast<int*>(take(slots * sizeof(int)));
         }
         args.big_rows = static_cast<int*>(take(static_cast<size_t>(rows) * sizeof(int)));
@@ -2167,12 +2553,16 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
         list.row0 = 0;
         list.rows = second.ov_cap;
         list.slices = second.slices;
-        list.partial = list.slices > 1 && k <= kListKMax ? static_cast<uint64_t*>(take(
+        list.partial = list.slices > 1 && k <= kListKMax && dp < kWideMin ? static_cast<uint64_t*>(take(
                            sizeof(uint64_t) * list.slices * static_cast<size_t>(list.rows) * k))
                                        : nullptr;
         list.cbuf = nullptr;
         list.ccount = nullptr;
         list.heavy = nullptr;
+        if (k > kListKMax || dp >= kWideMin) {
+            brute_batch = brute_batch_rows(n_points, list.rows);
+            brute_keys = static_cast<float*>(take(static_cast<size_t>(brute_batch) * n_points * sizeof(float)));
+        }
         // First-pass rows that find the second pass's queue full go there too.
         args.list_rows = second.ov_rows;
         args.list_count = second.ov_count;
@@ -2196,9 +2586,11 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
             // has to tell rows near kHeavyCount candidates from rows near n.
             const int m = std::min(n_points, std::clamp(8192 / dp, 64, 512));
             auto go = [&](auto D) {
-                heavy_rows_kernel<decltype(D)::value><<<(n_points + 255) / 256, 256, 0, stream>>>(
-                    ord->xs_ord, ord->order, tseed, guard, n_points, m, heavy, second.ov_rows,
-                    second.ov_count, second.ov_cap);
+                if constexpr (decltype(D)::value < kWideMin) {
+                    heavy_rows_kernel<decltype(D)::value><<<(n_points + 255) / 256, 256, 0, stream>>>(
+                        ord->xs_ord, ord->order, tseed, guard, n_points, m, heavy, second.ov_rows,
+                        second.ov_count, second.ov_cap);
+                }
                 return true;
             };
             for_dp<ScreenF32>(dp, go);
@@ -2210,12 +2602,12 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
             sb.wcol, padded_rows / 32, sb.wmax);
         CULOF_CHECK(hipGetLastError());
         auto go_bf16 = [&](auto D) {
-            launch_sym<ScreenBF16, decltype(D)::value>(args, sb, stream);
-            return true;
+            if constexpr (decltype(D)::value < kWideMin) launch_sym<ScreenBF16, decltype(D)::value>(args, sb, stream);
+            return decltype(D)::value < kWideMin;
         };
         auto go_f32 = [&](auto D) {
-            launch_sym<ScreenF32, decltype(D)::value>(args, sb, stream);
-            return true;
+            if constexpr (decltype(D)::value < kWideMin) launch_sym<ScreenF32, decltype(D)::value>(args, sb, stream);
+            return decltype(D)::value < kWideMin;
         };
         const bool ok = k <= kCollectKMax && (bf16 ? for_dp<ScreenBF16>(dp, go_bf16) : for_dp<ScreenF32>(dp, go_f32));
         if (!ok) throw std::invalid_argument("mfma_knn: unsupported (n_dims, k, form)");
@@ -2225,13 +2617,14 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
     }
     if (collect) {
         run(second);
-        if (k <= kListKMax) {
+        if (k <= kListKMax && dp < kWideMin) {
             run(list);
         } else {
-            // No list kernel holds this k: the rows queued for the list pass
-            // (heavy rows, second-pass overflow) take the exact radix select.
-            radix_knn(keyrows, stride, n_points, n_dims, k, 0, list.rows, list.qlist, list.row_count, nbr_idx,
-                      nbr_d2, kdist2, stream);
+            // No list kernel holds this k or width: the rows queued for the
+            // list pass (heavy rows, second-pass overflow) get their exact
+            // keys stored and the radix select over them.
+            brute_knn_rows(keyrows, stride, n_points, k, list.qlist, list.row_count, list.rows, brute_keys,
+                           brute_batch, nbr_idx, nbr_d2, kdist2, stream);
         }
     }
 }

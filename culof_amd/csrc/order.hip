@@ -75,8 +75,10 @@ __device__ void load_row(const float* __restrict__ row, float (&x)[DP]) {
     }
 }
 
-// Level-1 pivots: rows floor(t n / 64) of the key rows.
-template <int DP>
+// Level-1 pivots: rows floor(t n / 64) of the key rows. The cells use the
+// first DP features of rows of STRIDE floats (wider rows are ordered on
+// their first 128 features: the order only steers work).
+template <int DP, int STRIDE = DP>
 __global__ void __launch_bounds__(kBlock)
 level1_kernel(const float* __restrict__ xs, int n, int* __restrict__ cell,
               int* __restrict__ hist) {
@@ -84,7 +86,7 @@ level1_kernel(const float* __restrict__ xs, int n, int* __restrict__ cell,
     for (int c = threadIdx.x; c < kPiv * DP; c += kBlock) {
         const int t = c / DP;
         const long long row = static_cast<long long>(t) * n / kPiv;
-        piv[c] = xs[row * DP + (c - t * DP)];
+        piv[c] = xs[row * STRIDE + (c - t * DP)];
     }
     __shared__ int s_hist[kPiv];
     if (threadIdx.x < kPiv) s_hist[threadIdx.x] = 0;
@@ -92,7 +94,7 @@ level1_kernel(const float* __restrict__ xs, int n, int* __restrict__ cell,
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) {
         float x[DP];
-        load_row<DP>(xs + static_cast<size_t>(i) * DP, x);
+        load_row<DP>(xs + static_cast<size_t>(i) * STRIDE, x);
         const int c = nearest_pivot<DP>(x, piv, n < kPiv ? n : kPiv);
         cell[i] = c;
         atomicAdd(&s_hist[c], 1);
@@ -162,7 +164,7 @@ scatter_few_kernel(const int* __restrict__ key, int n, const int* __restrict__ o
 // Level-2 pivots of every level-1 cell: members at floor(t m / 64) of the
 // cell's range in the level-1 order (fewer when the cell is smaller).
 __global__ void __launch_bounds__(kBlock)
-level2_pivots_kernel(const float* __restrict__ xs, int dp, const int* __restrict__ order1,
+level2_pivots_kernel(const float* __restrict__ xs, int dp, int stride, const int* __restrict__ order1,
                      const int* __restrict__ offset1, const int* __restrict__ hist1,
                      float* __restrict__ piv2) {
     const int c = blockIdx.x;
@@ -172,14 +174,14 @@ level2_pivots_kernel(const float* __restrict__ xs, int dp, const int* __restrict
         float v = 0.0f;
         if (t < m) {
             const int member = order1[offset1[c] + static_cast<int>(static_cast<long long>(t) * m / kPiv)];
-            v = xs[static_cast<size_t>(member) * dp + (e - t * dp)];
+            v = xs[static_cast<size_t>(member) * stride + (e - t * dp)];
         }
         piv2[static_cast<size_t>(c) * kPiv * dp + e] = v;
     }
 }
 
 // Final key of each point: level-1 cell * 64 + nearest level-2 pivot of its cell.
-template <int DP>
+template <int DP, int STRIDE = DP>
 __global__ void __launch_bounds__(kBlock)
 level2_kernel(const float* __restrict__ xs, int n, const int* __restrict__ order1,
               const int* __restrict__ cell1, const int* __restrict__ hist1,
@@ -190,7 +192,7 @@ level2_kernel(const float* __restrict__ xs, int n, const int* __restrict__ order
     const int c = cell1[i];
     const int m = hist1[c];
     float x[DP];
-    load_row<DP>(xs + static_cast<size_t>(i) * DP, x);
+    load_row<DP>(xs + static_cast<size_t>(i) * STRIDE, x);
     const int s = nearest_pivot<DP>(x, piv2 + static_cast<size_t>(c) * kPiv * DP, m < kPiv ? m : kPiv);
     const int kk = c * kPiv + s;
     key[i] = kk;
@@ -220,10 +222,12 @@ gather_rows_kernel(const float* __restrict__ src, int dp, const int* __restrict_
 // queries per wave): exact keys against the W candidates of a window centred
 // on the block in the candidate order, streamed through LDS in chunks of
 // 64 RPL rows (lane l keys rows l, l + 64, ... of the chunk against the
-// wave's QPW queries; the next chunk is fetched into registers meanwhile),
+// wave's QPW queries; the next piece is fetched into registers meanwhile),
 // then each query's k-th smallest key by a bitwise search on the key bits.
 // DP: the features keyed (a multiple of 4); STRIDE: the row stride of xs_ord
 // (>= DP; the features past DP are zero padding, which adds exact zeros).
+// Rows wider than 128 features reach LDS in pieces of 128 features, the keys
+// carried across the pieces in ascending feature order (the same FMA chain).
 template <int DP, int STRIDE, int W, int QPW, int RPL>
 __global__ void __launch_bounds__(kBlock)
 seed_kernel(const float* __restrict__ xs_ord, const int* __restrict__ order, int n, int k,
@@ -231,14 +235,17 @@ seed_kernel(const float* __restrict__ xs_ord, const int* __restrict__ order, int
     constexpr int kQ = 4 * QPW;                // queries per block
     constexpr int kRows = 64 * RPL;            // candidate rows per chunk
     constexpr int kChunks = W / kRows;
-    constexpr int kPitch = DP + 4;             // LDS row pitch (floats)
-    constexpr int kF4 = kRows * DP / 4;        // float4s per chunk
+    constexpr int FC = DP < 128 ? DP : 128;    // features per LDS piece
+    constexpr int kPieces = DP / FC;           // pieces per chunk
+    constexpr int kPitch = FC + 4;             // LDS row pitch (floats)
+    constexpr int kF4 = kRows * FC / 4;        // float4s per piece
     constexpr int kLoads = (kF4 + kBlock - 1) / kBlock;
+    static_assert(DP % FC == 0, "whole pieces");
     // Wide rows: the wave's queries are read by scalar loads (their rows are
     // wave-uniform), so the LDS pipe serves only the candidate rows; from
     // LDS every step would read QPW query broadcasts besides one candidate.
     constexpr bool kScalarQ = DP >= 32;
-    __shared__ __attribute__((aligned(16))) float s_q[kScalarQ ? 1 : kQ][DP];
+    __shared__ __attribute__((aligned(16))) float s_q[kScalarQ ? 1 : kQ][kScalarQ ? 1 : DP];
     __shared__ __attribute__((aligned(16))) float s_c[kRows * kPitch];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -262,14 +269,17 @@ seed_kernel(const float* __restrict__ xs_ord, const int* __restrict__ order, int
         qrow[t] = xs_ord + static_cast<size_t>(__builtin_amdgcn_readfirstlane(pos)) * STRIDE;
     }
     float4 nxt[kLoads];
-    auto fetch = [&](int ch) {
+    // piece `pc` = chunk pc / kPieces, features [FC (pc % kPieces), + FC)
+    auto fetch = [&](int pc) {
+        const int ch = pc / kPieces;
+        const int f0 = (pc - ch * kPieces) * FC;
 #pragma unroll
         for (int u = 0; u < kLoads; ++u) {
             const int e = threadIdx.x + kBlock * u;
-            const int r = e / (DP / 4);
+            const int r = e / (FC / 4);
             const int c = ch * kRows + r;
             nxt[u] = (e < kF4 && c < w)
-                         ? reinterpret_cast<const float4*>(xs_ord + static_cast<size_t>(c0 + c) * STRIDE)[e - r * (DP / 4)]
+                         ? reinterpret_cast<const float4*>(xs_ord + static_cast<size_t>(c0 + c) * STRIDE + f0)[e - r * (FC / 4)]
                          : make_float4(0.f, 0.f, 0.f, 0.f);
         }
     };
@@ -280,49 +290,59 @@ seed_kernel(const float* __restrict__ xs_ord, const int* __restrict__ order, int
     uint32_t key[QPW][W / 64];
 #pragma unroll 1
     for (int ch = 0; ch < kChunks; ++ch) {
-        __syncthreads();
+        float e[RPL][QPW];
 #pragma unroll
-        for (int u = 0; u < kLoads; ++u) {
-            const int e = threadIdx.x + kBlock * u;
-            const int r = e / (DP / 4);
-            if (e < kF4) reinterpret_cast<float4*>(s_c + r * kPitch)[e - r * (DP / 4)] = nxt[u];
+        for (int rr = 0; rr < RPL; ++rr) {
+#pragma unroll
+            for (int t = 0; t < QPW; ++t) e[rr][t] = 0.0f;
         }
-        __syncthreads();
-        if (ch + 1 < kChunks) fetch(ch + 1);
+#pragma unroll 1
+        for (int pc = 0; pc < kPieces; ++pc) {
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < kLoads; ++u) {
+                const int e4 = threadIdx.x + kBlock * u;
+                const int r = e4 / (FC / 4);
+                if (e4 < kF4) reinterpret_cast<float4*>(s_c + r * kPitch)[e4 - r * (FC / 4)] = nxt[u];
+            }
+            __syncthreads();
+            if (ch * kPieces + pc + 1 < kChunks * kPieces) fetch(ch * kPieces + pc + 1);
+            const int q4 = pc * (FC / 4);  // the piece's first float4 of a query row
+#pragma unroll
+            for (int rr = 0; rr < RPL; ++rr) {
+                const int r = lane + 64 * rr;
+                const float4* row = reinterpret_cast<const float4*>(s_c + r * kPitch);
+#pragma unroll 4
+                for (int v = 0; v < FC / 4; ++v) {
+                    const float4 cv = row[v];
+#pragma unroll
+                    for (int t = 0; t < QPW; ++t) {
+                        float4 qv;
+                        if constexpr (kScalarQ) {
+                            qv = reinterpret_cast<const float4*>(qrow[t])[q4 + v];
+                        } else {
+                            qv = reinterpret_cast<const float4*>(&s_q[wave * QPW + t][0])[q4 + v];
+                        }
+                        float df = qv.x - cv.x;
+                        e[rr][t] = __builtin_fmaf(df, df, e[rr][t]);
+                        df = qv.y - cv.y;
+                        e[rr][t] = __builtin_fmaf(df, df, e[rr][t]);
+                        df = qv.z - cv.z;
+                        e[rr][t] = __builtin_fmaf(df, df, e[rr][t]);
+                        df = qv.w - cv.w;
+                        e[rr][t] = __builtin_fmaf(df, df, e[rr][t]);
+                    }
+                }
+            }
+        }
         uint32_t fresh[QPW][RPL];
 #pragma unroll
         for (int rr = 0; rr < RPL; ++rr) {
-            const int r = lane + 64 * rr;
-            float e[QPW];
-#pragma unroll
-            for (int t = 0; t < QPW; ++t) e[t] = 0.0f;
-            const float4* row = reinterpret_cast<const float4*>(s_c + r * kPitch);
-#pragma unroll 4
-            for (int v = 0; v < DP / 4; ++v) {
-                const float4 cv = row[v];
-#pragma unroll
-                for (int t = 0; t < QPW; ++t) {
-                    float4 qv;
-                    if constexpr (kScalarQ) {
-                        qv = reinterpret_cast<const float4*>(qrow[t])[v];
-                    } else {
-                        qv = reinterpret_cast<const float4*>(&s_q[wave * QPW + t][0])[v];
-                    }
-                    float df = qv.x - cv.x;
-                    e[t] = __builtin_fmaf(df, df, e[t]);
-                    df = qv.y - cv.y;
-                    e[t] = __builtin_fmaf(df, df, e[t]);
-                    df = qv.z - cv.z;
-                    e[t] = __builtin_fmaf(df, df, e[t]);
-                    df = qv.w - cv.w;
-                    e[t] = __builtin_fmaf(df, df, e[t]);
-                }
-            }
-            const int c = ch * kRows + r;
+            const int c = ch * kRows + lane + 64 * rr;
 #pragma unroll
             for (int t = 0; t < QPW; ++t) {
                 const bool self = c0 + c == p0 + wave * QPW + t;
-                fresh[t][rr] = (c >= w || self || e[t] != e[t]) ? kInfBits : __float_as_uint(e[t]);
+                fresh[t][rr] = (c >= w || self || e[rr][t] != e[rr][t]) ? kInfBits : __float_as_uint(e[rr][t]);
             }
         }
 #pragma unroll
@@ -355,18 +375,18 @@ seed_kernel(const float* __restrict__ xs_ord, const int* __restrict__ order, int
     }
 }
 
-template <int DP>
+template <int DP, int STRIDE = DP>
 void launch_order(const float* xs, int n, int* cell1, int* hist1, int* offset1, int* cursor1,
                   int* order1, float* piv2, int* key, int* hist, int* offset, int* cursor,
                   int* order, int* inv, hipStream_t s) {
     const unsigned grid = static_cast<unsigned>((n + kBlock - 1) / kBlock);
     zero_fill(hist1, sizeof(int) * kPiv, s);
     zero_fill(hist, sizeof(int) * kKeys, s);
-    level1_kernel<DP><<<grid, kBlock, 0, s>>>(xs, n, cell1, hist1);
+    level1_kernel<DP, STRIDE><<<grid, kBlock, 0, s>>>(xs, n, cell1, hist1);
     scan_bins_kernel<<<1, kBlock, 0, s>>>(hist1, kPiv, offset1, cursor1);
     scatter_few_kernel<<<grid, kBlock, 0, s>>>(cell1, n, offset1, cursor1, order1);
-    level2_pivots_kernel<<<kPiv, kBlock, 0, s>>>(xs, DP, order1, offset1, hist1, piv2);
-    level2_kernel<DP><<<grid, kBlock, 0, s>>>(xs, n, order1, cell1, hist1, piv2, key, hist);
+    level2_pivots_kernel<<<kPiv, kBlock, 0, s>>>(xs, DP, STRIDE, order1, offset1, hist1, piv2);
+    level2_kernel<DP, STRIDE><<<grid, kBlock, 0, s>>>(xs, n, order1, cell1, hist1, piv2, key, hist);
     scan_bins_kernel<<<1, kBlock, 0, s>>>(hist, kKeys, offset, cursor);
     scatter_kernel<<<grid, kBlock, 0, s>>>(key, n, offset, cursor, order);
     inverse_kernel<<<grid, kBlock, 0, s>>>(order, n, inv);
@@ -374,10 +394,20 @@ void launch_order(const float* xs, int n, int* cell1, int* hist1, int* offset1, 
 
 template <int DP, int STRIDE = DP>
 void launch_seed(const float* xs_ord, const int* order, int n, int k, float* tseed, hipStream_t s) {
+    constexpr int RPL = DP <= 8 ? 8 : (DP <= 32 ? 2 : 1);  // about 16 KiB of rows per chunk
+    if (k > kListKMax) {
+        // Large k: the k-th key of a 2048-candidate window (a window of 1024
+        // leaves seeds several times the true k-th rank, and the candidate
+        // sets grow with them).
+        constexpr int W = 2048;
+        constexpr int QPW = 2;
+        const unsigned grid = static_cast<unsigned>((n + 4 * QPW - 1) / (4 * QPW));
+        seed_kernel<DP, STRIDE, W, QPW, RPL><<<grid, kBlock, 0, s>>>(xs_ord, order, n, k, tseed);
+        return;
+    }
     // Narrow rows afford a wider window (tighter seeds) for the same work.
     constexpr int W = (DP <= 8 || DP > 32) ? 1024 : 2048;
     constexpr int QPW = DP <= 32 ? 2 : 4;
-    constexpr int RPL = DP <= 8 ? 8 : (DP <= 32 ? 2 : 1);  // about 16 KiB of rows per chunk
     const unsigned grid = static_cast<unsigned>((n + 4 * QPW - 1) / (4 * QPW));
     seed_kernel<DP, STRIDE, W, QPW, RPL><<<grid, kBlock, 0, s>>>(xs_ord, order, n, k, tseed);
 }
@@ -395,7 +425,9 @@ size_t order_scratch_bytes(int n, int dp) {
     return b;
 }
 
-bool order_supported(int dp) { return dp == 4 || dp == 8 || dp == 16 || dp == 32 || dp == 64 || dp == 128; }
+bool order_supported(int dp) {
+    return dp == 4 || dp == 8 || dp == 16 || dp == 32 || dp == 64 || dp == 128 || dp == 256 || dp == 512;
+}
 
 CandidateOrder build_order(const float* xs, int n, int dp, void* scratch, hipStream_t s) {
     if (!order_supported(dp)) throw std::invalid_argument("build_order: unsupported row stride");
@@ -428,7 +460,9 @@ CandidateOrder build_order(const float* xs, int n, int dp, void* scratch, hipStr
         case 16: launch_order<16>(xs, n, cell1, hist1, offset1, cursor1, order1, piv2, key, hist, offset, cursor, o.order, o.inv, s); break;
         case 32: launch_order<32>(xs, n, cell1, hist1, offset1, cursor1, order1, piv2, key, hist, offset, cursor, o.order, o.inv, s); break;
         case 64: launch_order<64>(xs, n, cell1, hist1, offset1, cursor1, order1, piv2, key, hist, offset, cursor, o.order, o.inv, s); break;
-        default: launch_order<128>(xs, n, cell1, hist1, offset1, cursor1, order1, piv2, key, hist, offset, cursor, o.order, o.inv, s); break;
+        case 128: launch_order<128>(xs, n, cell1, hist1, offset1, cursor1, order1, piv2, key, hist, offset, cursor, o.order, o.inv, s); break;
+        case 256: launch_order<128, 256>(xs, n, cell1, hist1, offset1, cursor1, order1, piv2, key, hist, offset, cursor, o.order, o.inv, s); break;
+        default: launch_order<128, 512>(xs, n, cell1, hist1, offset1, cursor1, order1, piv2, key, hist, offset, cursor, o.order, o.inv, s); break;
     }
     const size_t cells4 = nn * (dp / 4);
     gather_rows_kernel<<<static_cast<unsigned>((cells4 + kBlock - 1) / kBlock), kBlock, 0, s>>>(
@@ -454,7 +488,9 @@ void seed_thresholds(const CandidateOrder& o, int n, int key_dp, int k, float* t
         case 16: launch_seed<16>(o.xs_ord, o.order, n, k, tseed, s); break;
         case 32: launch_seed<32>(o.xs_ord, o.order, n, k, tseed, s); break;
         case 64: launch_seed<64>(o.xs_ord, o.order, n, k, tseed, s); break;
-        default: launch_seed<128>(o.xs_ord, o.order, n, k, tseed, s); break;
+        case 128: launch_seed<128>(o.xs_ord, o.order, n, k, tseed, s); break;
+        case 256: launch_seed<256>(o.xs_ord, o.order, n, k, tseed, s); break;
+        default: launch_seed<512>(o.xs_ord, o.order, n, k, tseed, s); break;
     }
     CULOF_CHECK(hipGetLastError());
 }

@@ -21,6 +21,7 @@
 #include "internal.hpp"
 
 #include <algorithm>
+#include <stdexcept>
 
 namespace culof {
 namespace detail {
@@ -117,6 +118,24 @@ struct RecomputedKeys {
     }
 };
 
+// f(key(j)) for every j in [0, n) of this thread's share, four keys loaded
+// before any is used (histogram counts do not depend on the order).
+template <class Keys, class F>
+__device__ __forceinline__ void for_each_key(const Keys& key, int n, F&& f) {
+    int j = threadIdx.x;
+    for (; j + 3 * kBlock < n; j += 4 * kBlock) {
+        const uint32_t u0 = key(j);
+        const uint32_t u1 = key(j + kBlock);
+        const uint32_t u2 = key(j + 2 * kBlock);
+        const uint32_t u3 = key(j + 3 * kBlock);
+        f(u0);
+        f(u1);
+        f(u2);
+        f(u3);
+    }
+    for (; j < n; j += kBlock) f(key(j));
+}
+
 template <class Keys>
 __device__ void select_row(const Keys& key, RadixShared& sh, int n, int k, int out_col,
                            int out_stride, int* __restrict__ nbr_idx, float* __restrict__ nbr_d2,
@@ -127,27 +146,25 @@ __device__ void select_row(const Keys& key, RadixShared& sh, int n, int k, int o
     // pass 0: top 11 bits
     for (int b = threadIdx.x; b < kBins; b += kBlock) sh.hist[b] = 0;
     __syncthreads();
-    for (int j = threadIdx.x; j < n; j += kBlock) atomicAdd(&sh.hist[key(j) >> 21], 1);
+    for_each_key(key, n, [&](uint32_t u) { atomicAdd(&sh.hist[u >> 21], 1); });
     __syncthreads();
     refine(sh, kBins, 21, k, prefix, below);
 
     // pass 1: middle 11 bits of keys matching the top 11
     for (int b = threadIdx.x; b < kBins; b += kBlock) sh.hist[b] = 0;
     __syncthreads();
-    for (int j = threadIdx.x; j < n; j += kBlock) {
-        const uint32_t u = key(j);
+    for_each_key(key, n, [&](uint32_t u) {
         if ((u >> 21) == (prefix >> 21)) atomicAdd(&sh.hist[(u >> 10) & 0x7FFu], 1);
-    }
+    });
     __syncthreads();
     refine(sh, kBins, 10, k - below, prefix, below);
 
     // pass 2: low 10 bits of keys matching the top 22
     for (int b = threadIdx.x; b < kBins; b += kBlock) sh.hist[b] = 0;
     __syncthreads();
-    for (int j = threadIdx.x; j < n; j += kBlock) {
-        const uint32_t u = key(j);
+    for_each_key(key, n, [&](uint32_t u) {
         if ((u >> 10) == (prefix >> 10)) atomicAdd(&sh.hist[u & 0x3FFu], 1);
-    }
+    });
     __syncthreads();
     refine(sh, 1024, 0, k - below, prefix, below);
 
@@ -240,6 +257,74 @@ radix_query_kernel(const float* __restrict__ qrows, const float* __restrict__ xs
     }
 }
 
+// Exact keys of a batch of listed query rows against every key row, for the
+// stored select below: keys[b * n + j] for list entries r0 + b of
+// row_list[0 .. min(*count, cap)), b < batch. A block stages QB query rows in
+// LDS and keys 256 candidates (one per thread) against all of them, the
+// candidate row read once as float4s; the chain is the exact FMA chain of
+// every other path (zero padding adds exact zeros), the diagonal and NaN +inf.
+template <int QB>
+__global__ void __launch_bounds__(kBlock)
+brute_keys_kernel(const float* __restrict__ xs, int stride, int n, const int* __restrict__ row_list,
+                  const int* __restrict__ count, int cap, int r0, int batch, float* __restrict__ keys) {
+    extern __shared__ float s_rows[];  // QB * stride floats
+    const int total = min(*count, cap);
+    const int qb0 = r0 + static_cast<int>(blockIdx.y) * QB;
+    const int end = min(total, r0 + batch);
+    if (qb0 >= end) return;  // whole block
+    const int nq = min(QB, end - qb0);
+    for (int e = threadIdx.x; e < nq * stride; e += kBlock) {
+        const int q = e / stride;
+        s_rows[e] = xs[static_cast<size_t>(row_list[qb0 + q]) * stride + (e - q * stride)];
+    }
+    __syncthreads();
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n) return;
+    float acc[QB];
+#pragma unroll
+    for (int q = 0; q < QB; ++q) acc[q] = 0.0f;
+    const float4* c4 = reinterpret_cast<const float4*>(xs + static_cast<size_t>(j) * stride);
+    const float4* q4 = reinterpret_cast<const float4*>(s_rows);
+    const int f4 = stride / 4;
+#pragma unroll 2
+    for (int v = 0; v < f4; ++v) {
+        const float4 c = c4[v];
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+            const float4 a = q4[q * f4 + v];
+            float df = a.x - c.x;
+            acc[q] = __builtin_fmaf(df, df, acc[q]);
+            df = a.y - c.y;
+            acc[q] = __builtin_fmaf(df, df, acc[q]);
+            df = a.z - c.z;
+            acc[q] = __builtin_fmaf(df, df, acc[q]);
+            df = a.w - c.w;
+            acc[q] = __builtin_fmaf(df, df, acc[q]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+        if (q < nq) {
+            const int qi = row_list[qb0 + q];
+            const float key = (j == qi || acc[q] != acc[q]) ? __uint_as_float(kInfBits) : acc[q];
+            keys[static_cast<size_t>(qb0 - r0 + q) * n + j] = key;
+        }
+    }
+}
+
+// The stored select over a batch of brute_keys_kernel rows; outputs at the
+// rows' original indices (stride n).
+__global__ void __launch_bounds__(kBlock)
+radix_list_stored_kernel(const float* __restrict__ keys, int n, int k, const int* __restrict__ row_list,
+                         const int* __restrict__ count, int cap, int r0, int* __restrict__ nbr_idx,
+                         float* __restrict__ nbr_d2, float* __restrict__ kdist2) {
+    __shared__ RadixShared sh;
+    const int r = r0 + static_cast<int>(blockIdx.x);
+    if (r >= min(*count, cap)) return;
+    const StoredKeys key{keys + static_cast<size_t>(blockIdx.x) * n};
+    select_row(key, sh, n, k, row_list[r], n, nbr_idx, nbr_d2, kdist2);
+}
+
 }  // namespace
 
 void radix_query_knn(const float* qrows, const float* keyrows, int stride, int n_points, int n_dims,
@@ -264,6 +349,38 @@ void radix_knn(const float* keyrows, int stride, int n_points, int n_dims, int k
         keyrows, stride, n_dims, n_points, k, row0, rows, row_list, row_count, nbr_idx, nbr_d2,
         kdist2);
     CULOF_CHECK(hipGetLastError());
+}
+
+int brute_batch_rows(int n_points, int cap) {
+    const size_t per_row = static_cast<size_t>(std::max(n_points, 1)) * sizeof(float);
+    const int rows = static_cast<int>(std::min<size_t>(kBruteKeyBytes / per_row, 1 << 20));
+    return std::max(1, std::min(cap, std::max(rows, 64)));
+}
+
+void brute_knn_rows(const float* keyrows, int stride, int n_points, int k, const int* row_list,
+                    const int* row_count, int cap, float* keys, int batch, int* nbr_idx, float* nbr_d2,
+                    float* kdist2, hipStream_t stream) {
+    ProfileScope prof("brute_rows", stream);
+    if (cap <= 0) return;
+    if (stride % 4 != 0) throw std::invalid_argument("brute_knn_rows: row stride must be whole float4s");
+    const unsigned cgrid = static_cast<unsigned>((n_points + kBlock - 1) / kBlock);
+    // Batches past the device count exit at once (no host read-back).
+    for (int r0 = 0; r0 < cap; r0 += batch) {
+        const int b = std::min(batch, cap - r0);
+        if (stride <= 128) {
+            constexpr int QB = 32;
+            brute_keys_kernel<QB><<<dim3(cgrid, (b + QB - 1) / QB), kBlock, sizeof(float) * QB * stride, stream>>>(
+                keyrows, stride, n_points, row_list, row_count, cap, r0, b, keys);
+        } else {
+            constexpr int QB = 16;
+            brute_keys_kernel<QB><<<dim3(cgrid, (b + QB - 1) / QB), kBlock, sizeof(float) * QB * stride, stream>>>(
+                keyrows, stride, n_points, row_list, row_count, cap, r0, b, keys);
+        }
+        CULOF_CHECK(hipGetLastError());
+        radix_list_stored_kernel<<<b, kBlock, 0, stream>>>(keys, n_points, k, row_list, row_count, cap, r0,
+                                                          nbr_idx, nbr_d2, kdist2);
+        CULOF_CHECK(hipGetLastError());
+    }
 }
 
 void radix_select_rows(const float* keys, int rows, int n_cols, int k, int* nbr_idx,

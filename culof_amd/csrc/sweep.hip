@@ -456,7 +456,8 @@ bool dispatch(int dp, int k, const SweepArgs* args, hipStream_t stream) {
 }  // namespace
 
 int padded_dims(int n_dims) {
-    for (int p : {4, 8, 16, 32, 64, 128}) {
+    // Powers of two up to 512: the widths the matrix-core pipeline is built for.
+    for (int p : {4, 8, 16, 32, 64, 128, 256, 512}) {
         if (n_dims <= p) return p;
     }
     return (n_dims + 3) / 4 * 4;

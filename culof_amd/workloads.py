@@ -46,7 +46,8 @@ def clusters(n: int = 200_000, d: int = 8, n_clusters: int = 5, spread: float = 
 
 
 def lowrank(n: int = 100_000, d: int = 128, rank: int = 32, n_clusters: int = 16,
-            seed: int = 0) -> np.ndarray:
+            seed: int = 0, return_outliers: bool = False):
+    """X, or (X, mask of the planted N(0, 8^2) outlier rows) with ``return_outliers``."""
     rng = np.random.default_rng(seed)
     n_out = int(round(n * 0.01))
     blocks = []
@@ -58,7 +59,11 @@ def lowrank(n: int = 100_000, d: int = 128, rank: int = 32, n_clusters: int = 16
         blocks.append(c + (z * s) @ basis.T + 0.1 * rng.standard_normal((m, d)))
     blocks.append(rng.normal(0.0, 8.0, size=(n_out, d)))
     X = np.concatenate(blocks).astype(np.float32)
-    return np.ascontiguousarray(X[rng.permutation(n)])
+    perm = rng.permutation(n)
+    X = np.ascontiguousarray(X[perm])
+    if return_outliers:
+        return X, perm >= n - n_out
+    return X
 
 
 def grad(n: int = 50_000, d: int = 32, seed: int = 0) -> tuple[np.ndarray, np.ndarray]:

@@ -3,8 +3,9 @@
 The reference's selection is independent of k (include/culof.h:39-41,
 src/cuda/knn.cu:119-232 serve any k). Here the candidate order, the seeds,
 the collect passes and the selects serve k up to 256; rows the list pass
-would take (heavy rows, second-pass overflow) and every row of a run without
-the candidate order take the exact radix select. Every comparison is bit for
+would take (heavy rows, second-pass overflow) get their exact keys stored and
+the stored radix select, and every row of a run without the candidate order
+takes the exact radix select. Every comparison is bit for
 bit against the oracle.
 """
 
@@ -35,7 +36,7 @@ def _profiled(fn):
         out = fn()
     finally:
         _native.profile_enable(False)
-    scopes = ("mfma_collect", "collect_select", "big_select", "radix_knn", "mfma_sweep")
+    scopes = ("mfma_collect", "collect_select", "big_select", "radix_knn", "mfma_sweep", "brute_rows")
     return out, {s: _native.profile_read(s)[1] for s in scopes}
 
 
@@ -108,10 +109,10 @@ def test_symmetric_pass_large_k(monkeypatch, n, d, k, form):
         np.testing.assert_array_equal(_bits(sym[2][r]), _bits(ok[0]), err_msg=f"row {r} kdist2")
 
 
-def test_heavy_rows_large_k_take_radix():
-    """At n > 65,536 wide outliers are heavy rows; with k > 64 they go to the
-    radix select instead of the list pass. Those rows and sampled inliers match
-    the oracle."""
+def test_heavy_rows_large_k_take_stored_keys():
+    """At n > 65,536 wide outliers are heavy rows; with k > 64 they get stored
+    exact keys and the stored radix select instead of the list pass. Those
+    rows and sampled inliers match the oracle."""
     from _gpu import knn
 
     rng = np.random.default_rng(5)
@@ -120,7 +121,7 @@ def test_heavy_rows_large_k_take_radix():
     far = rng.choice(n, 300, replace=False)
     X[far] = rng.uniform(-60, 60, (far.size, d)).astype(np.float32)
     got, launches = _profiled(lambda: knn(X, k))
-    assert launches["radix_knn"] > 0 and launches["mfma_sweep"] == 0
+    assert launches["brute_rows"] > 0 and launches["mfma_sweep"] == 0
     for r in np.r_[far[:6], rng.choice(n, 6, replace=False)]:
         oi, od, ok = oracle.knn_rows(X, k, int(r), 1)
         np.testing.assert_array_equal(got[0][:, r], oi[0], err_msg=f"row {r}")
