@@ -63,8 +63,8 @@ FP32_PEAK_TFLOPS = 157.3      # MI355X dense FP32 (matrix and packed vector)
 FP32_VALU_TFLOPS = 78.6       # non-packed FP32 VALU (packed FP32 ops are disabled at compile time)
 HBM_PEAK_GBS = 8000.0
 REFERENCE_PTS_S = 1.42e5      # reference on RTX A6000 at n=200k, d=8, k=20 (its README.md:50)
-KERNELS = ("center_partials", "center_apply", "order", "seed", "heavy_rows", "screen_prep",
-           "mfma_collect", "scatter_pairs", "collect_select", "big_select",
+KERNELS = ("center_partials", "center_apply", "order", "seed", "order_split", "heavy_rows", "screen_prep",
+           "sym_prune", "mfma_collect", "scatter_pairs", "collect_select", "big_select",
            "second_collect",
            "second_select", "mfma_sweep", "merge_slices", "sweep_knn", "brute_rows", "radix_knn", "tie_break",
            "tie_collect", "tie_band", "lrd", "score")
@@ -348,6 +348,7 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     _native.profile_enable(False)
     total_kernel = {name: _native.profile_read(name) for name in KERNELS}
+    tile_pairs = _native.profile_tile_pairs()
     dominant = max(total_kernel, key=lambda name: total_kernel[name][0])
     sweep_ms, sweep_launches = total_kernel[dominant]
     _native.profile_reset()
@@ -358,7 +359,9 @@ def main() -> int:
     # ---- roofline of the dominant kernel (the symmetric collect pass) ----
     avg_sweep_s = sweep_ms / max(sweep_launches, 1) / 1e3
     dp = -(-d // 4) * 4 if d > 4 else 4
-    executed = executed_flops(n, dp, _native.FORM_PLAIN)
+    all_pairs_flops = executed_flops(n, dp, _native.FORM_PLAIN)
+    # the pass screens only the tile pairs its pruning keeps (counted on the device)
+    executed = (tile_pairs / max(sweep_launches, 1) * 32.0 * 32.0 * 2.0 * dp) if tile_pairs else all_pairs_flops
     achieved = executed / avg_sweep_s / 1e12
     effective = 2.0 * d * float(n) * float(n) / avg_sweep_s / 1e12
     roofline = {
@@ -371,11 +374,14 @@ def main() -> int:
         "kernel": dominant,
         "avg_launch_ms": round(avg_sweep_s * 1e3, 4),
         "flops_per_launch": executed,
+        "tile_pairs_per_launch": round(tile_pairs / max(sweep_launches, 1)) if tile_pairs else None,
+        "kept_fraction_of_unordered_pairs": round(executed / all_pairs_flops, 4),
         "effective_tflops": round(effective, 3),
         "effective_frac": round(effective / FP32_PEAK_TFLOPS, 4),
         "note": "achieved = FP32 matrix-core flops the symmetric collect pass executes per "
-                "launch (2*dp per pair of every 128-row block pair it visits, each unordered "
-                "pair once; flops_per_launch) / its average HIP-event launch time, against the "
+                "launch (2*dp per row pair of every 32x32 tile pair it screens: each unordered "
+                "pair at most once, and only the tile pairs its box pruning keeps, counted on "
+                "the device; flops_per_launch) / its average HIP-event launch time, against the "
                 "FP32 matrix peak; effective_* count 2*d*n^2 (every ordered pair, the reference's "
                 "Gram contraction) over the same time; traffic = HBM bytes per launch from the "
                 "committed FETCH_SIZE/WRITE_SIZE summary",

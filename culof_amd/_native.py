@@ -46,6 +46,7 @@ SIGNATURES = {
     "culof_profile_enable": (_c_int, [_c_int]),
     "culof_profile_reset": (_c_int, []),
     "culof_profile_read": (_c_int, [_cp, ctypes.POINTER(_c_double), ctypes.POINTER(_c_long)]),
+    "culof_profile_tile_pairs": (_c_int, [ctypes.POINTER(ctypes.c_longlong)]),
 }
 
 _lib = None
@@ -106,3 +107,11 @@ def profile_read(name: str) -> tuple[float, int]:
     launches = ctypes.c_long()
     check(lib().culof_profile_read(name.encode(), ctypes.byref(ms), ctypes.byref(launches)))
     return ms.value, launches.value
+
+
+def profile_tile_pairs() -> int:
+    """Query x candidate tile pairs (32 x 32 rows) the symmetric passes screened
+    while timing was on, since the last reset."""
+    v = ctypes.c_longlong()
+    check(lib().culof_profile_tile_pairs(ctypes.byref(v)))
+    return v.value

@@ -66,6 +66,29 @@ std::vector<ProfRecord> g_prof;
 
 }  // namespace
 
+// The tile-pair work counter of culof_profile_tile_pairs: one device word
+// per device, allocated when timing is first enabled on it.
+unsigned long long* profile_tile_counter_for(int device, bool create) {
+    static std::mutex mu;
+    static std::vector<std::pair<int, unsigned long long*>> counters;
+    std::lock_guard<std::mutex> lock(mu);
+    for (auto& c : counters)
+        if (c.first == device) return c.second;
+    if (!create) return nullptr;
+    unsigned long long* p = nullptr;
+    CULOF_CHECK(hipMalloc(&p, sizeof(unsigned long long)));
+    CULOF_CHECK(hipMemset(p, 0, sizeof(unsigned long long)));
+    counters.emplace_back(device, p);
+    return p;
+}
+
+unsigned long long* profile_tile_counter() {
+    if (!g_prof_on.load(std::memory_order_relaxed)) return nullptr;
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return nullptr;
+    return profile_tile_counter_for(device, false);
+}
+
 ProfileScope::ProfileScope(const char* name, hipStream_t stream) : stream_(stream) {
     if (!g_prof_on.load(std::memory_order_relaxed)) return;
     ProfRecord rec;
@@ -413,8 +436,14 @@ void knn_rows(const Prepared& p, int n, int d, int k, int form, int row0, int ro
             ord = build_order(p.xs, n, p.stride,
                               scratch.get<unsigned char>(order_scratch_bytes(n, p.stride)), s);
             tseed = scratch.get<float>(static_cast<size_t>(n));
-            if (row0 == 0 && rows == n) qorder = ord.order;
             seed_thresholds(ord, n, padded_dims(d), k, tseed, s);
+            if (row0 == 0 && rows == n) {
+                // A full run takes its queries in the order too; its rows with
+                // the largest seeds go last, which tightens the tile bounds
+                // the symmetric pass prunes with.
+                order_large_seeds_last(ord, n, tseed, s);
+                qorder = ord.order;
+            }
         }
     }
     for (int r0 = 0; r0 < rows; r0 += tile) {
@@ -814,8 +843,14 @@ int culof_choose_tile_rows(int n_points, size_t budget_bytes) {
 }
 
 int culof_profile_enable(int on) {
-    g_prof_on.store(on != 0);
-    return CULOF_OK;
+    return guarded([&] {
+        if (on) {
+            int device = 0;
+            CULOF_CHECK(hipGetDevice(&device));
+            (void)profile_tile_counter_for(device, true);
+        }
+        g_prof_on.store(on != 0);
+    });
 }
 
 int culof_profile_reset(void) {
@@ -827,6 +862,27 @@ int culof_profile_reset(void) {
             (void)hipEventDestroy(r.stop);
         }
         g_prof.clear();
+        int device = 0;
+        if (hipGetDevice(&device) == hipSuccess) {
+            if (unsigned long long* c = profile_tile_counter_for(device, false)) {
+                CULOF_CHECK(hipDeviceSynchronize());
+                CULOF_CHECK(hipMemset(c, 0, sizeof(unsigned long long)));
+            }
+        }
+    });
+}
+
+int culof_profile_tile_pairs(long long* tile_pairs) {
+    return guarded([&] {
+        if (!tile_pairs) throw std::invalid_argument("null argument");
+        int device = 0;
+        CULOF_CHECK(hipGetDevice(&device));
+        unsigned long long v = 0;
+        if (unsigned long long* c = profile_tile_counter_for(device, false)) {
+            CULOF_CHECK(hipDeviceSynchronize());
+            CULOF_CHECK(hipMemcpy(&v, c, sizeof(v), hipMemcpyDeviceToHost));
+        }
+        *tile_pairs = static_cast<long long>(v);
     });
 }
 

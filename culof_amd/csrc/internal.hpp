@@ -59,6 +59,9 @@ inline int screen_form(int form) {
 int padded_dims(int n_dims);
 
 // ---- kernel timing hooks (see culof_profile_* in culof_c.h) ----
+// The device's tile-pair work counter while timing is on, else null.
+unsigned long long* profile_tile_counter();
+unsigned long long* profile_tile_counter_for(int device, bool create);
 struct ProfileScope {
     ProfileScope(const char* name, hipStream_t stream);
     ~ProfileScope();
@@ -176,6 +179,12 @@ struct CandidateOrder {
     int* inv = nullptr;       // original index -> position
     float* xs_ord = nullptr;  // key rows in candidate order (stride dp)
     int dp = 0;
+    const float* xs = nullptr;  // the key rows the order was built from (natural order)
+    // work space left free once the order is built (n, n, 4096 and 64 ints)
+    int* tmp_order = nullptr;
+    int* tmp_counts = nullptr;
+    int* tmp_hist = nullptr;
+    int* tmp_small = nullptr;
 };
 bool order_supported(int dp);
 size_t order_scratch_bytes(int n, int dp);
@@ -184,6 +193,11 @@ CandidateOrder build_order(const float* xs, int n, int dp, void* scratch, hipStr
 // a window of candidates near it in the order). key_dp: the padded feature
 // count of the keys (<= o.dp; the rest of each row is zero padding).
 void seed_thresholds(const CandidateOrder& o, int n, int key_dp, int k, float* tseed, hipStream_t s);
+// Moves the rows whose seeds are the largest (at most n / 100, by a histogram
+// of the seeds) to the end of the order, each part keeping its order, and
+// rebuilds inv and xs_ord. Changes no result: it tightens the tile bounds the
+// symmetric pass prunes with.
+void order_large_seeds_last(CandidateOrder& o, int n, const float* tseed, hipStream_t s);
 
 // ---- ties.hip: FP64 tie-break over a (k+1)-neighbour result ----
 // Relative band (1 + g') / (1 - g') of FP32 keys that FP64 may reorder.

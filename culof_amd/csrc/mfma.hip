@@ -402,7 +402,8 @@ template <class Op, int DP, bool kSym, int QT = 1, class Visit>
 __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename Op::template Frag<DP>* bq,
                                              int j, int h, unsigned char* stage, int t_begin, int t_end,
                                              int wrap, const float* __restrict__ wcol,
-                                             const float2* __restrict__ wmax, Visit&& visit) {
+                                             const float2* __restrict__ wmax, Visit&& visit,
+                                             const int* __restrict__ clist = nullptr, int nlist = 0) {
     static_assert(QT == 1 || (QT == 2 && kSym), "two query tiles per wave: symmetric pass only");
     using Frag = typename Op::template Frag<DP>;
     using Elem = typename Op::Elem;
@@ -427,7 +428,10 @@ __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename O
     float4* s_w = s_m + 2 * kChunkM4;              // kSym: column thresholds
     float2* s_x = reinterpret_cast<float2*>(s_w + 2 * kChunkM4);  // kSym: per-tile maxima
 
-    const int n_chunks = (t_end - t_begin + CT - 1) / CT;
+    // Chunks of CT tiles from t_begin, or (clist) only the listed chunk
+    // starts, nlist of them (the pruned symmetric pass).
+    const int n_chunks = clist ? nlist : (t_end - t_begin + CT - 1) / CT;
+    auto chunk_start = [&](int ch) { return clist ? clist[ch] : t_begin + ch * CT; };
     const float4* ga = static_cast<const float4*>(p.xa);
     const float4* gm = reinterpret_cast<const float4*>(p.madj);
     const float4* gw = reinterpret_cast<const float4*>(wcol);
@@ -548,15 +552,15 @@ __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename O
         if constexpr (kDma) __builtin_amdgcn_s_waitcnt(0);
     };
     if (n_chunks > 0) {
-        fetch(t_begin, 0);
+        fetch(chunk_start(0), 0);
         stage_chunk(0);
     }
     dma_wait();
     __syncthreads();
     for (int ch = 0; ch < n_chunks; ++ch) {
         const int buf = ch & 1;
-        const int t0 = t_begin + ch * CT;
-        if (ch + 1 < n_chunks) fetch(t0 + CT, buf ^ 1);  // next chunk in flight
+        const int t0 = chunk_start(ch);
+        if (ch + 1 < n_chunks) fetch(chunk_start(ch + 1), buf ^ 1);  // next chunk in flight
         const float4* sa = s_a + buf * kLdsChunkF4;
         const float4* sm = s_m + buf * kChunkM4;
         if constexpr (kSym) {
@@ -1094,13 +1098,20 @@ constexpr int sym_query_tiles() {
     return (std::is_same_v<Op, ScreenBF16> && DP >= 64) ? 2 : 1;
 }
 
+// Candidate tiles per LDS chunk of the symmetric pass (stream_range's CT).
+template <class Op, int DP>
+constexpr int sym_chunk_tiles() {
+    return sym_query_tiles<Op, DP>() == 2 ? ChunkShape<Op, DP>::kDmaTiles : ChunkShape<Op, DP>::kTiles;
+}
+
 template <class Op, int DP>
 __global__ void __launch_bounds__(64 * kTilesPerBlock)
 __attribute__((amdgpu_waves_per_eu(DP >= 128 || sym_query_tiles<Op, DP>() == 2 ? 2 : 4, 8)))
 mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2* __restrict__ wmax,
                         uint64_t* __restrict__ pool, int* __restrict__ chunk_region,
                         int* __restrict__ chunk_count, int* __restrict__ next_chunk, int max_chunks,
-                        int* __restrict__ col_over) {
+                        int* __restrict__ col_over, const int* __restrict__ live_chunks,
+                        const int* __restrict__ live_count, int live_stride, int n_slices) {
     using Frag = typename Op::template Frag<DP>;
     constexpr int QT = sym_query_tiles<Op, DP>();
     constexpr int kBlockTiles = kTilesPerBlock * QT;
@@ -1111,9 +1122,14 @@ mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2
     const int wave = threadIdx.x >> 6;
     const int j = lane & 31;
     const int h = lane >> 5;
-    const int nb = static_cast<int>(gridDim.x);
-    const int B = static_cast<int>(blockIdx.x);
-    const int ns = static_cast<int>(gridDim.y);
+    // A 1-D grid of nb x ns blocks, dispatched from the last block of
+    // positions backwards (the rows with the largest seeds sit there and
+    // keep the most chunks after pruning: started first, they do not trail),
+    // the slices of a block together.
+    const int ns = n_slices;
+    const int nb = static_cast<int>(gridDim.x) / ns;
+    const int B = nb - 1 - static_cast<int>(blockIdx.x) / ns;
+    const int ys = static_cast<int>(blockIdx.x) % ns;  // this block's slice
     const int qtile0 = (B * kTilesPerBlock + wave) * QT;
 
     int q[QT];
@@ -1129,7 +1145,7 @@ mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2
     if (*p.guard) {
 #pragma unroll
         for (int t = 0; t < QT; ++t)
-            if (valid[t] && h == 0 && blockIdx.y == 0) queue_push(p.fb_rows, p.fb_count, p.fb_cap, qi[t]);
+            if (valid[t] && h == 0 && ys == 0) queue_push(p.fb_rows, p.fb_count, p.fb_cap, qi[t]);
         return;
     }
 
@@ -1149,8 +1165,8 @@ mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2
     }
     const int h_b = (nb - 1) / 2 + ((nb % 2 == 0 && B < nb / 2) ? 1 : 0);
     const long long span = h_b + 1;  // offsets 0..h_b
-    const int o0 = static_cast<int>(span * blockIdx.y / ns);
-    const int o1 = static_cast<int>(span * (blockIdx.y + 1) / ns);
+    const int o0 = static_cast<int>(span * ys / ns);
+    const int o1 = static_cast<int>(span * (ys + 1) / ns);
     const int wrap = kBlockTiles * nb;
     const int v_col = kBlockTiles * (B + 1);  // tiles of other blocks from here
     const int cap = p.cap;
@@ -1161,7 +1177,7 @@ mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2
     int fill[QT];     // words used in it
 #pragma unroll
     for (int t = 0; t < QT; ++t) {
-        qbuf[t] = p.cbuf + (static_cast<size_t>(valid[t] ? q[t] : 0) * nslot + blockIdx.y) * cap;
+        qbuf[t] = p.cbuf + (static_cast<size_t>(valid[t] ? q[t] : 0) * nslot + ys) * cap;
         cnt[t] = 0;
         chunk[t] = -1;
         fill[t] = kChunkWords;
@@ -1173,7 +1189,7 @@ mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2
         if (lane == 0) id = atomicAdd(next_chunk, 1);
         chunk[t] = __shfl(id, 0, 64);
         fill[t] = 0;
-        if (lane == 0 && chunk[t] < max_chunks) chunk_region[chunk[t]] = (qtile0 + t) * ns + static_cast<int>(blockIdx.y);
+        if (lane == 0 && chunk[t] < max_chunks) chunk_region[chunk[t]] = (qtile0 + t) * ns + ys;
     };
 
     auto visit = [&](const floatx16& acc, int v, float wm, const float4* wrow, int t) {
@@ -1230,13 +1246,159 @@ mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2
             }
         }
     };
+    // Only the chunks sym_live_chunks_kernel kept (when it ran).
+    const int region = B * ns + ys;
+    const int* clist = live_chunks ? live_chunks + static_cast<size_t>(region) * live_stride : nullptr;
+    const int nlist = live_chunks ? live_count[region] : 0;
     stream_range<Op, DP, true, QT>(p, bq, j, h, smem, kBlockTiles * (B + o0), kBlockTiles * (B + o1), wrap,
-                                   wcol, wmax, visit);
+                                   wcol, wmax, visit, clist, nlist);
 #pragma unroll
     for (int t = 0; t < QT; ++t) {
-        if (valid[t] && h == 0) p.ccount[static_cast<size_t>(q[t]) * nslot + blockIdx.y] = cnt[t];
+        if (valid[t] && h == 0) p.ccount[static_cast<size_t>(q[t]) * nslot + ys] = cnt[t];
         if (lane == 0 && chunk[t] >= 0 && chunk[t] < max_chunks) chunk_count[chunk[t]] = fill[t];
     }
+}
+
+// ---- pruning of the symmetric pass by tile bounds ----
+//
+// A pair of rows matters to the pass only when its exact key can be at most
+// the query's seed (row test) or the candidate's (column test); rows flagged
+// heavy test nothing. Per tile of 32 positions: the bounding box of its key
+// rows and the largest seed among its rows that test. A chunk of candidate
+// tiles whose box lies farther from the block's query box than both the
+// block's and the chunk's largest seeds (the lower bound, rounded, scaled
+// down by (2 DP + 16) 2^-23 to cover its own rounding and the exact keys')
+// holds no such pair and is skipped. Results do not change.
+template <int DP>
+__global__ void __launch_bounds__(256)
+tile_bounds_kernel(const float* __restrict__ xs_ord, const int* __restrict__ order,
+                   const float* __restrict__ tseed, const int* __restrict__ heavy, int n, int tiles,
+                   float* __restrict__ box, float* __restrict__ tmax) {
+    const int e = blockIdx.x * 256 + threadIdx.x;  // (tile, feature)
+    if (e >= tiles * DP) return;
+    const int t = e / DP;
+    const int f = e - t * DP;
+    float lo = __uint_as_float(kInfBits);
+    float hi = -__uint_as_float(kInfBits);
+    float tm = -__uint_as_float(kInfBits);
+    for (int r = 0; r < 32; ++r) {
+        const int pos = t * 32 + r;
+        if (pos >= n) break;
+        const float v = xs_ord[static_cast<size_t>(pos) * DP + f];
+        lo = __builtin_fminf(lo, v);  // NaN coordinates drop out (their keys are +inf)
+        hi = __builtin_fmaxf(hi, v);
+        if (f == 0 && !heavy[pos]) tm = __builtin_fmaxf(tm, tseed[order[pos]]);
+    }
+    box[(static_cast<size_t>(t) * 2) * DP + f] = lo;
+    box[(static_cast<size_t>(t) * 2 + 1) * DP + f] = hi;
+    if (f == 0) tmax[t] = tm;
+}
+
+// For block B and slice y of the symmetric pass: the starts of its candidate
+// chunks (CT tiles from kBlockTiles (B + o0)) that may hold a pair, in
+// ascending order, and their count. A chunk stays when one of its tiles
+// does: the tile's box within reach of the block's box for the larger of the
+// block's and the tile's largest seeds (one thread per tile, 256 tiles per
+// round). The block's own tiles (lower bound 0) always stay.
+template <int DP, int CT, int kBlockTiles>
+__global__ void __launch_bounds__(256)
+sym_live_chunks_kernel(const float* __restrict__ box, const float* __restrict__ tmax, int nb, int ns,
+                       int stride, int* __restrict__ live, int* __restrict__ live_count,
+                       unsigned long long* __restrict__ visited) {
+    static_assert(256 % CT == 0, "whole chunks per round");
+    __shared__ __attribute__((aligned(16))) float s_lo[DP];
+    __shared__ __attribute__((aligned(16))) float s_hi[DP];
+    __shared__ float s_tq;
+    __shared__ int s_ok[256];
+    __shared__ int s_wave[4];
+    __shared__ int s_tiles;  // candidate tiles of the kept chunks
+    const int B = blockIdx.x;
+    const int y = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int h_b = (nb - 1) / 2 + ((nb % 2 == 0 && B < nb / 2) ? 1 : 0);
+    const long long span = h_b + 1;
+    const int o0 = static_cast<int>(span * y / ns);
+    const int o1 = static_cast<int>(span * (y + 1) / ns);
+    const int wrap = kBlockTiles * nb;
+    const int t_begin = kBlockTiles * (B + o0);
+    const int t_end = kBlockTiles * (B + o1);
+    const int nc = (t_end - t_begin + CT - 1) / CT;
+    for (int f = tid; f < DP; f += 256) {
+        float lo = __uint_as_float(kInfBits);
+        float hi = -__uint_as_float(kInfBits);
+        for (int t = kBlockTiles * B; t < kBlockTiles * (B + 1); ++t) {
+            lo = __builtin_fminf(lo, box[(static_cast<size_t>(t) * 2) * DP + f]);
+            hi = __builtin_fmaxf(hi, box[(static_cast<size_t>(t) * 2 + 1) * DP + f]);
+        }
+        s_lo[f] = lo;
+        s_hi[f] = hi;
+    }
+    if (tid == 0) {
+        float tq = -__uint_as_float(kInfBits);
+        for (int t = kBlockTiles * B; t < kBlockTiles * (B + 1); ++t) tq = __builtin_fmaxf(tq, tmax[t]);
+        s_tq = tq;
+        s_tiles = 0;
+    }
+    __syncthreads();
+    const float shrink = 1.0f - (2.0f * DP + 16.0f) * 2.0f * kU;
+    int* out = live + static_cast<size_t>(B * ns + y) * stride;
+    constexpr int kChunksPerRound = 256 / CT;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    int total = 0;
+    for (int c0 = 0; c0 < nc; c0 += kChunksPerRound) {
+        const int v = t_begin + c0 * CT + tid;
+        int ok = 0;
+        if (v < t_end) {
+            const size_t pt = static_cast<size_t>(v >= wrap ? v - wrap : v);
+            const float4* lo4 = reinterpret_cast<const float4*>(box + (pt * 2) * DP);
+            const float4* hi4 = reinterpret_cast<const float4*>(box + (pt * 2 + 1) * DP);
+            const float4* blo4 = reinterpret_cast<const float4*>(s_lo);
+            const float4* bhi4 = reinterpret_cast<const float4*>(s_hi);
+            float lb = 0.0f;
+#pragma unroll 8
+            for (int f4 = 0; f4 < DP / 4; ++f4) {
+                const float4 l = lo4[f4], hh = hi4[f4], bl = blo4[f4], bh = bhi4[f4];
+                float g = __builtin_fmaxf(0.0f, __builtin_fmaxf(l.x - bh.x, bl.x - hh.x));
+                lb = __builtin_fmaf(g, g, lb);
+                g = __builtin_fmaxf(0.0f, __builtin_fmaxf(l.y - bh.y, bl.y - hh.y));
+                lb = __builtin_fmaf(g, g, lb);
+                g = __builtin_fmaxf(0.0f, __builtin_fmaxf(l.z - bh.z, bl.z - hh.z));
+                lb = __builtin_fmaf(g, g, lb);
+                g = __builtin_fmaxf(0.0f, __builtin_fmaxf(l.w - bh.w, bl.w - hh.w));
+                lb = __builtin_fmaf(g, g, lb);
+            }
+            const float thr = __builtin_fmaxf(s_tq, tmax[pt]);
+            ok = !(lb * shrink > thr);  // NaN keeps
+        }
+        s_ok[tid] = ok;
+        __syncthreads();
+        const int c = c0 + tid;
+        bool keep = false;
+        if (tid < kChunksPerRound && c < nc) {
+#pragma unroll
+            for (int e = 0; e < CT; ++e) keep |= s_ok[tid * CT + e] != 0;
+        }
+        // compact in ascending chunk order
+        const uint64_t bal = __ballot(keep);
+        if (lane == 0) s_wave[wave] = __popcll(bal);
+        __syncthreads();
+        int before = total;
+        for (int w = 0; w < wave; ++w) before += s_wave[w];
+        if (keep) {
+            out[before + lanes_below(bal)] = t_begin + c * CT;
+            if (visited) atomicAdd(&s_tiles, min(CT, t_end - (t_begin + c * CT)));
+        }
+        total += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        __syncthreads();
+    }
+    if (tid == 0) live_count[B * ns + y] = total;
+    if (visited && tid == 0) atomicAdd(visited, static_cast<unsigned long long>(s_tiles) * kBlockTiles);
+}
+
+// The work counter for an unpruned symmetric pass: every tile pair it visits.
+__global__ void add_counter_kernel(unsigned long long* __restrict__ c, unsigned long long v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(c, v);
 }
 
 // Column words of each pool chunk (written by query tile t, slice y) into
@@ -2197,19 +2359,43 @@ struct SymBuffers {
     int max_chunks;
     int* col_over;
     int slices;  // S: offset slices (the rows have 2S buffer slices)
+    // pruning: per-tile boxes and largest seeds, per (block, slice) the live chunks
+    float* box = nullptr;
+    float* tmax = nullptr;
+    int* live = nullptr;
+    int* live_count = nullptr;
+    int live_stride = 0;
 };
 
 // Symmetric first pass of a full run, the column pairs into their rows, then
 // the selects.
 template <class Op, int DP>
 void launch_sym(const MfmaArgs& a, const SymBuffers& sb, hipStream_t stream) {
-    constexpr int rows_per_block = 32 * kTilesPerBlock * sym_query_tiles<Op, DP>();
+    constexpr int kBlockTiles = kTilesPerBlock * sym_query_tiles<Op, DP>();
+    constexpr int rows_per_block = 32 * kBlockTiles;
     const int nb = (a.n + rows_per_block - 1) / rows_per_block;
+    if (sb.live) {
+        ProfileScope prof("sym_prune", stream);
+        const int tiles = kBlockTiles * nb;
+        tile_bounds_kernel<DP><<<(tiles * DP + 255) / 256, 256, 0, stream>>>(
+            a.xs_cand, a.cand, a.tseed, a.heavy, a.n, tiles, sb.box, sb.tmax);
+        sym_live_chunks_kernel<DP, sym_chunk_tiles<Op, DP>(), kBlockTiles>
+            <<<dim3(nb, sb.slices), 256, 0, stream>>>(sb.box, sb.tmax, nb, sb.slices, sb.live_stride, sb.live,
+                                                      sb.live_count, profile_tile_counter());
+        CULOF_CHECK(hipGetLastError());
+    } else if (unsigned long long* counter = profile_tile_counter()) {
+        unsigned long long pairs = 0;
+        for (int B = 0; B < nb; ++B) {
+            const int h_b = (nb - 1) / 2 + ((nb % 2 == 0 && B < nb / 2) ? 1 : 0);
+            pairs += static_cast<unsigned long long>(h_b + 1) * kBlockTiles * kBlockTiles;
+        }
+        add_counter_kernel<<<1, 64, 0, stream>>>(counter, pairs);
+    }
     {
         ProfileScope prof("mfma_collect", stream);
-        mfma_sym_collect_kernel<Op, DP><<<dim3(nb, sb.slices), 64 * kTilesPerBlock, 0, stream>>>(
+        mfma_sym_collect_kernel<Op, DP><<<nb * sb.slices, 64 * kTilesPerBlock, 0, stream>>>(
             a, sb.wcol, sb.wmax, sb.pool, sb.chunk_region, sb.chunk_count, sb.next_chunk, sb.max_chunks,
-            sb.col_over);
+            sb.col_over, sb.live, sb.live_count, sb.live_stride, sb.slices);
     }
     {
         ProfileScope prof("scatter_pairs", stream);
@@ -2420,6 +2606,53 @@ int sym_slices(int n_points, int qt) {
     return std::max(1, std::min(std::min(by_waves, by_len), 16));
 }
 
+// CULOF_PRUNE=0 turns the symmetric pass's tile pruning off (the same
+// results; for measurement).
+bool prune_enabled() {
+    const char* env = std::getenv("CULOF_PRUNE");
+    return !(env && env[0] == '0');
+}
+
+// Sizes of the pruning buffers of a symmetric pass over n_points rows with ns
+// slices: tiles (boxes and largest seeds) and live-chunk entries per
+// (block, slice).
+struct PruneShape {
+    int nb = 0;
+    int tiles = 0;
+    int stride = 0;
+};
+
+PruneShape prune_shape(int n_points, int dp, int form, int ns) {
+    PruneShape ps;
+    const int qt = sym_qt(dp, form);
+    const int block_tiles = kTilesPerBlock * qt;
+    int ct = 1;
+    auto chunk = [&](auto D) {
+        constexpr int DPv = decltype(D)::value;
+        if constexpr (DPv < kWideMin) {
+            ct = form == kFormBF16 ? sym_chunk_tiles<ScreenBF16, DPv>() : sym_chunk_tiles<ScreenF32, DPv>();
+        }
+        return true;
+    };
+    if (form == kFormBF16) {
+        for_dp<ScreenBF16>(dp, chunk);
+    } else {
+        for_dp<ScreenF32>(dp, chunk);
+    }
+    ps.nb = (n_points + 32 * block_tiles - 1) / (32 * block_tiles);
+    ps.tiles = block_tiles * ps.nb;
+    const int span = (ps.nb - 1) / 2 + 2;
+    const int per = (span + ns - 1) / ns;
+    ps.stride = (block_tiles * per + ct - 1) / ct + 1;
+    return ps;
+}
+
+size_t prune_bytes(int n_points, int dp, int form, int ns) {
+    const PruneShape ps = prune_shape(n_points, dp, form, ns);
+    return static_cast<size_t>(ps.tiles) * (2 * dp + 1) * sizeof(float) +
+           static_cast<size_t>(ps.nb) * ns * (ps.stride + 1) * sizeof(int) + 4 * 256;
+}
+
 size_t mfma_scratch_bytes(int n_points, int dp, int form, int rows, int k, bool collect) {
     const size_t padded = padded_positions(n_points);
     const size_t rows_bytes = form == kFormBF16 ? 2 * sizeof(uint16_t) : sizeof(float);
@@ -2434,7 +2667,7 @@ size_t mfma_scratch_bytes(int n_points, int dp, int form, int rows, int k, bool 
         const size_t sym = static_cast<size_t>(rows) * (2 * ss * (first_cap(k) + 1) + 2) * sizeof(uint32_t) +
                            padded * sizeof(float) + padded / 32 * sizeof(float2) +
                            static_cast<size_t>(sym_chunks(n_points)) * (kChunkWords * sizeof(uint64_t) + 2 * sizeof(int)) +
-                           2048;
+                           prune_bytes(n_points, dp, form, static_cast<int>(ss)) + 2048;
         // (the symmetric pass serves full runs only)
         const bool sym_run = rows == n_points && n_points >= kSymMinRows;
         const size_t first = std::max(slots * (first_cap(k) + 1) * sizeof(uint32_t), sym_run ? sym : 0) + 256;
@@ -2512,6 +2745,14 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
             sb.chunk_count = static_cast<int*>(take(chunks * sizeof(int)));
             sb.next_chunk = static_cast<int*>(take(sizeof(int)));
             sb.col_over = static_cast<int*>(take(static_cast<size_t>(rows) * sizeof(int)));
+            if (prune_enabled()) {
+                const PruneShape ps = prune_shape(n_points, dp, form, sb.slices);
+                sb.box = static_cast<float*>(take(static_cast<size_t>(ps.tiles) * 2 * dp * sizeof(float)));
+                sb.tmax = static_cast<float*>(take(static_cast<size_t>(ps.tiles) * sizeof(float)));
+                sb.live_stride = ps.stride;
+                sb.live = static_cast<int*>(take(static_cast<size_t>(ps.nb) * sb.slices * ps.stride * sizeof(int)));
+                sb.live_count = static_cast<int*>(take(static_cast<size_t>(ps.nb) * sb.slices * sizeof(int)));
+            }
             zero_fill(args.ccount, slots * sizeof(int), stream);
             zero_fill(sb.next_chunk, sizeof(int), stream);
             zero_fill(sb.col_over, static_cast<size_t>(rows) * sizeof(int), stream);

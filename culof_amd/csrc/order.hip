@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <stdexcept>
+#include <utility>
 
 namespace culof {
 namespace detail {
@@ -375,6 +376,120 @@ seed_kernel(const float* __restrict__ xs_ord, const int* __restrict__ order, int
     }
 }
 
+// ---- rows with the largest seeds last ----
+//
+// A seed far above its neighbours' (an outlier, whose k-th neighbour is far
+// away) widens the bounding box and the threshold of its tile, and the
+// symmetric pass can then rule out few tile pairs by their boxes. Such rows
+// are moved to the end of the candidate order, each part keeping its order:
+// at most about n / 100 of them, those whose seed falls in the top bins of a
+// histogram over the seeds' bit patterns (1/16 octave bins).
+constexpr int kSeedBins = 4096;  // bits >> 19 of a non-negative float
+constexpr int kSplitSpan = 1024;  // positions per block of the partition
+
+// Seeds of one data set crowd into a few bins: counts go to a histogram in
+// LDS first (kSplitSpan seeds per block), then its nonzero bins to memory.
+__global__ void __launch_bounds__(kBlock)
+seed_hist_kernel(const float* __restrict__ tseed, int n, int* __restrict__ hist) {
+    __shared__ int s_hist[kSeedBins];
+    for (int b = threadIdx.x; b < kSeedBins; b += kBlock) s_hist[b] = 0;
+    __syncthreads();
+    for (int e = 0; e < kSplitSpan / kBlock; ++e) {
+        const int i = blockIdx.x * kSplitSpan + e * kBlock + threadIdx.x;
+        if (i < n) atomicAdd(&s_hist[__float_as_uint(tseed[i]) >> 19], 1);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kSeedBins; b += kBlock)
+        if (s_hist[b]) atomicAdd(&hist[b], s_hist[b]);
+}
+
+// small[0] = first bin of the large seeds, small[1] = their count: the
+// histogram is staged in LDS, then one thread walks it down from the top.
+__global__ void __launch_bounds__(kBlock)
+seed_split_bin_kernel(const int* __restrict__ hist, int target, int* __restrict__ small) {
+    __shared__ int s_hist[kSeedBins];
+    for (int b = threadIdx.x; b < kSeedBins; b += kBlock) s_hist[b] = hist[b];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    int bin = kSeedBins;
+    int count = 0;
+    while (bin > 0 && count + s_hist[bin - 1] <= target) count += s_hist[--bin];
+    small[0] = bin;
+    small[1] = count;
+}
+
+__device__ __forceinline__ bool large_seed(const float* __restrict__ tseed, const int* __restrict__ order,
+                                           int p, int bin) {
+    return static_cast<int>(__float_as_uint(tseed[order[p]]) >> 19) >= bin;
+}
+
+// counts[b] = positions of block b (kSplitSpan of them) that stay in front.
+__global__ void __launch_bounds__(kBlock)
+split_count_kernel(const float* __restrict__ tseed, const int* __restrict__ order, int n,
+                   const int* __restrict__ small, int* __restrict__ counts) {
+    __shared__ int s_sum[kBlock / 64];
+    const int bin = small[0];
+    int c = 0;
+    for (int e = 0; e < kSplitSpan / kBlock; ++e) {
+        const int p = blockIdx.x * kSplitSpan + e * kBlock + threadIdx.x;
+        c += p < n && !large_seed(tseed, order, p, bin);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0) s_sum[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kBlock / 64; ++w) t += s_sum[w];
+        counts[blockIdx.x] = t;
+    }
+}
+
+// Stable partition: the positions of block b that stay go to offsets[b] + their
+// rank among them; the others to (n - n_large) + their rank among those.
+__global__ void __launch_bounds__(kBlock)
+split_scatter_kernel(const float* __restrict__ tseed, const int* __restrict__ order, int n,
+                     const int* __restrict__ small, const int* __restrict__ offsets,
+                     int* __restrict__ out) {
+    __shared__ int s_wave[kBlock / 64];
+    const int bin = small[0];
+    const int front_total = n - small[1];
+    constexpr int kPer = kSplitSpan / kBlock;  // consecutive positions per thread
+    const int p0 = blockIdx.x * kSplitSpan + threadIdx.x * kPer;
+    bool stay[kPer];
+    int mine = 0;
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const int p = p0 + e;
+        stay[e] = p < n && !large_seed(tseed, order, p, bin);
+        mine += stay[e];
+    }
+    // exclusive scan of `mine` over the block
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    int x = mine;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    int before = offsets[blockIdx.x];
+    for (int w = 0; w < wave; ++w) before += s_wave[w];
+    before += x - mine;  // kept positions before p0 in the whole order
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const int p = p0 + e;
+        if (p >= n) break;
+        if (stay[e]) {
+            out[before++] = order[p];
+        } else {
+            out[front_total + (p - before)] = order[p];
+        }
+    }
+}
+
 template <int DP, int STRIDE = DP>
 void launch_order(const float* xs, int n, int* cell1, int* hist1, int* offset1, int* cursor1,
                   int* order1, float* piv2, int* key, int* hist, int* offset, int* cursor,
@@ -453,6 +568,12 @@ CandidateOrder build_order(const float* xs, int n, int dp, void* scratch, hipStr
     int* cursor = static_cast<int*>(take(kKeys * sizeof(int)));
     o.xs_ord = static_cast<float*>(take(nn * dp * sizeof(float)));
     o.dp = dp;
+    o.xs = xs;
+    // free once the order is built: work space of order_large_seeds_last
+    o.tmp_order = order1;
+    o.tmp_counts = key;
+    o.tmp_hist = hist;
+    o.tmp_small = offset1;
     ProfileScope prof("order", s);
     switch (dp) {
         case 4: launch_order<4>(xs, n, cell1, hist1, offset1, cursor1, order1, piv2, key, hist, offset, cursor, o.order, o.inv, s); break;
@@ -469,6 +590,29 @@ CandidateOrder build_order(const float* xs, int n, int dp, void* scratch, hipStr
         xs, dp, o.order, n, o.xs_ord);
     CULOF_CHECK(hipGetLastError());
     return o;
+}
+
+void order_large_seeds_last(CandidateOrder& o, int n, const float* tseed, hipStream_t s) {
+    ProfileScope prof("order_split", s);
+    const int target = n / 100;
+    if (target < 1) return;
+    const unsigned grid = static_cast<unsigned>((n + kBlock - 1) / kBlock);
+    const int blocks = (n + kSplitSpan - 1) / kSplitSpan;
+    zero_fill(o.tmp_hist, sizeof(int) * kSeedBins, s);
+    seed_hist_kernel<<<blocks, kBlock, 0, s>>>(tseed, n, o.tmp_hist);
+    seed_split_bin_kernel<<<1, kBlock, 0, s>>>(o.tmp_hist, target, o.tmp_small);
+    int* counts = o.tmp_counts;
+    int* offsets = o.tmp_counts + blocks;
+    split_count_kernel<<<blocks, kBlock, 0, s>>>(tseed, o.order, n, o.tmp_small, counts);
+    scan_bins_kernel<<<1, kBlock, 0, s>>>(counts, blocks, offsets, o.tmp_counts + 2 * blocks);
+    split_scatter_kernel<<<blocks, kBlock, 0, s>>>(tseed, o.order, n, o.tmp_small, offsets, o.tmp_order);
+    CULOF_CHECK(hipGetLastError());
+    std::swap(o.order, o.tmp_order);
+    inverse_kernel<<<grid, kBlock, 0, s>>>(o.order, n, o.inv);
+    const size_t cells4 = static_cast<size_t>(n) * (o.dp / 4);
+    gather_rows_kernel<<<static_cast<unsigned>((cells4 + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+        o.xs, o.dp, o.order, n, o.xs_ord);
+    CULOF_CHECK(hipGetLastError());
 }
 
 void seed_thresholds(const CandidateOrder& o, int n, int key_dp, int k, float* tseed, hipStream_t s) {

@@ -189,6 +189,11 @@ int culof_choose_tile_rows(int n_points, size_t budget_bytes);
 int culof_profile_enable(int on);
 int culof_profile_reset(void);
 int culof_profile_read(const char* name, double* total_ms, long* launches);
+/* Work counter beside the timers: pairs of 32-row tiles (a query tile and a
+ * candidate tile, 32 x 32 row pairs) the symmetric collect passes screened on
+ * the current device while timing was enabled, since the last reset (the
+ * pruned pass skips the rest). Synchronises the device. */
+int culof_profile_tile_pairs(long long* tile_pairs);
 
 #ifdef __cplusplus
 }

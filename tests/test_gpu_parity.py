@@ -267,6 +267,33 @@ def test_symmetric_pass_switch_gives_same_bits(monkeypatch, n, d, k, form):
         _assert_same_bits(sym[2][r], ok[0], f"row {r} kdist2")
 
 
+@pytest.mark.parametrize("n,d,k,form", [(40_000, 8, 20, 0), (36_000, 16, 10, 1), (33_000, 64, 31, 0),
+                                         (35_840, 64, 16, 1), (34_000, 128, 50, 1), (35_000, 32, 15, 0),
+                                         (50_000, 4, 7, 0), (45_000, 8, 100, 0)])
+def test_pruned_symmetric_pass_gives_same_bits(monkeypatch, n, d, k, form):
+    """The symmetric pass skips candidate chunks whose tile boxes lie farther
+    from the block's than every seed that could use them (rows with the
+    largest seeds ordered last). CULOF_PRUNE=0 (every chunk) gives the same
+    bits, on clustered data with far outliers, and sampled rows match the
+    oracle."""
+    from _gpu import knn
+    from culof_amd import workloads
+
+    rng = np.random.default_rng(n + d + k)
+    X = workloads.clusters(n=n, d=d, seed=n % 13)
+    far = rng.choice(n, max(1, n // 200), replace=False)
+    X[far] = X[far] * np.float32(6.0)
+    pruned = knn(X, k, form=form)
+    monkeypatch.setenv("CULOF_PRUNE", "0")
+    full = knn(X, k, form=form)
+    for a, b, name in zip(pruned, full, ("idx", "d2", "kdist2")):
+        np.testing.assert_array_equal(a, b, err_msg=name)
+    for r in np.r_[far[:3], rng.choice(n, 5, replace=False)]:
+        oi, od, ok = oracle.knn_rows(X, k, int(r), 1)
+        np.testing.assert_array_equal(pruned[0][:, r], oi[0], err_msg=f"row {r}")
+        _assert_same_bits(pruned[2][r], ok[0], f"row {r} kdist2")
+
+
 @pytest.mark.parametrize("form", [0, 1], ids=["fp32", "bf16"])
 def test_overflow_guard_in_symmetric_pass(form):
     """Norms beyond FLT_MAX / 16 set the device guard: the symmetric pass and
