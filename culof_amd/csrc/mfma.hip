@@ -383,7 +383,11 @@ struct MfmaArgs {
     int* fb_count;
     int fb_cap;          // entries fb_rows holds
     const int* heavy;    // symmetric pass: 1 for rows (by position) queued for the list pass up front
-    const int* qpos;     // wide rows: original index -> candidate position (null: identity)
+    const int* qpos;     // original index -> candidate position (null: identity)
+    // First-pass wave select of a run in candidate order: each row's list goes
+    // to stage[local * (2k + 2)] (k indices, k keys, the k-th key, 1 = emitted)
+    // and unstage_kernel writes the transposed outputs; null: written directly.
+    int* stage;
 };
 
 // Streams candidate tiles [t_begin, t_end) in chunks of CT tiles through a
@@ -1570,7 +1574,7 @@ __device__ __forceinline__ uint64_t collected_word(const MfmaArgs& p, const floa
 
 // The k smallest words of a row, in any order in sel[0, k) (k <= 64): written
 // in the reference's slot order by the calling wave (all 64 lanes).
-__device__ void emit_selected(const MfmaArgs& p, const uint64_t* sel, int qi, int lane) {
+__device__ void emit_selected(const MfmaArgs& p, const uint64_t* sel, int qi, int lane, int local = -1) {
     // Called by a whole wave. Lane f holds word f; the k-th word is their
     // maximum. Sorting the lanes by (group, index) -- group 0 for keys below
     // the k-th, 1 for keys equal to it, 2 for lanes past k -- puts every word
@@ -1607,6 +1611,18 @@ __device__ void emit_selected(const MfmaArgs& p, const uint64_t* sel, int qi, in
             }
         }
     }
+    if (local >= 0) {  // staged: one contiguous record per row
+        int* rec = p.stage + static_cast<size_t>(local) * (2 * k + 2);
+        if (valid) {
+            rec[lane] = static_cast<int>(static_cast<uint32_t>(s));
+            rec[k + lane] = static_cast<int>(key);
+        }
+        if (lane == 0) {
+            rec[2 * k] = static_cast<int>(kth);
+            rec[2 * k + 1] = 1;
+        }
+        return;
+    }
     if (valid) {
         const size_t ns = static_cast<size_t>(p.n);
         p.nbr_idx[static_cast<size_t>(lane) * ns + qi] = static_cast<int>(static_cast<uint32_t>(s));
@@ -1618,7 +1634,7 @@ __device__ void emit_selected(const MfmaArgs& p, const uint64_t* sel, int qi, in
 // k smallest of the `total` words in s_words (WPL words per lane), emitted.
 template <int WPL>
 __device__ void select_words(const MfmaArgs& p, const uint64_t* s_words, int total, int qi, int lane,
-                             uint64_t* s_sel) {
+                             uint64_t* s_sel, int local = -1) {
     uint64_t w[WPL];
 #pragma unroll
     for (int u = 0; u < WPL; ++u) {
@@ -1676,14 +1692,15 @@ __device__ void select_words(const MfmaArgs& p, const uint64_t* s_words, int tot
         off += __popcll(mask);
     }
     wave_lds_sync();
-    emit_selected(p, s_sel, qi, lane);
+    emit_selected(p, s_sel, qi, lane, local);
 }
 
 // emit_selected for 64 < k <= 64 KW: the same slot order, the bitonic network
 // over 64 KW words (strides below 64 between lanes by shuffles, the others
 // between registers of one lane).
 template <int KW>
-__device__ __forceinline__ void emit_selected_wide(const MfmaArgs& p, const uint64_t* sel, int qi, int lane) {
+__device__ __forceinline__ void emit_selected_wide(const MfmaArgs& p, const uint64_t* sel, int qi, int lane,
+                                                   int local) {
     // Word e = 64 u + lane sits in register u of its lane; the k-th word is
     // their maximum. Sorting the words by (group, index) -- group 0 for keys
     // below the k-th, 1 for keys equal to it, 2 past k -- puts every word at
@@ -1754,6 +1771,22 @@ __device__ __forceinline__ void emit_selected_wide(const MfmaArgs& p, const uint
             }
         }
     }
+    if (local >= 0) {  // staged record (see emit_selected)
+        int* rec = p.stage + static_cast<size_t>(local) * (2 * k + 2);
+#pragma unroll
+        for (int u = 0; u < KW; ++u) {
+            const int e = 64 * u + lane;
+            if (e < k) {
+                rec[e] = static_cast<int>(static_cast<uint32_t>(s[u]));
+                rec[k + e] = static_cast<int>(key[u]);
+            }
+        }
+        if (lane == 0) {
+            rec[2 * k] = static_cast<int>(kth);
+            rec[2 * k + 1] = 1;
+        }
+        return;
+    }
     const size_t ns = static_cast<size_t>(p.n);
 #pragma unroll
     for (int u = 0; u < KW; ++u) {
@@ -1767,11 +1800,12 @@ __device__ __forceinline__ void emit_selected_wide(const MfmaArgs& p, const uint
 }
 
 // The emit for 64 < k <= kCollectKMax (the selects' WIDE instantiations).
-__device__ __forceinline__ void emit_wide(const MfmaArgs& p, const uint64_t* sel, int qi, int lane) {
+__device__ __forceinline__ void emit_wide(const MfmaArgs& p, const uint64_t* sel, int qi, int lane,
+                                          int local = -1) {
     if (p.k <= 128) {
-        emit_selected_wide<2>(p, sel, qi, lane);
+        emit_selected_wide<2>(p, sel, qi, lane, local);
     } else {
-        emit_selected_wide<4>(p, sel, qi, lane);
+        emit_selected_wide<4>(p, sel, qi, lane, local);
     }
 }
 
@@ -1779,7 +1813,7 @@ __device__ __forceinline__ void emit_wide(const MfmaArgs& p, const uint64_t* sel
 // s_words[0, k) (read whole by then) for the wide emit.
 template <int WPL>
 __device__ __forceinline__ void select_words_wide(const MfmaArgs& p, uint64_t* s_words, int total, int qi,
-                                                  int lane) {
+                                                  int lane, int local) {
     uint64_t w[WPL];
 #pragma unroll
     for (int u = 0; u < WPL; ++u) {
@@ -1838,19 +1872,20 @@ __device__ __forceinline__ void select_words_wide(const MfmaArgs& p, uint64_t* s
         off += __popcll(mask);
     }
     wave_lds_sync();
-    emit_wide(p, s_words, qi, lane);
+    emit_wide(p, s_words, qi, lane, local);
 }
 
 // The wave select for k > 64 (the WIDE instantiation of the select kernel).
-__device__ __forceinline__ void wide_select(const MfmaArgs& p, uint64_t* s_words, int total, int qi, int lane) {
+__device__ __forceinline__ void wide_select(const MfmaArgs& p, uint64_t* s_words, int total, int qi, int lane,
+                                            int local) {
     if (total <= 128) {
-        select_words_wide<2>(p, s_words, total, qi, lane);
+        select_words_wide<2>(p, s_words, total, qi, lane, local);
     } else if (total <= 256) {
-        select_words_wide<4>(p, s_words, total, qi, lane);
+        select_words_wide<4>(p, s_words, total, qi, lane, local);
     } else if (total <= 512) {
-        select_words_wide<8>(p, s_words, total, qi, lane);
+        select_words_wide<8>(p, s_words, total, qi, lane, local);
     } else {
-        select_words_wide<16>(p, s_words, total, qi, lane);
+        select_words_wide<16>(p, s_words, total, qi, lane, local);
     }
 }
 
@@ -1876,6 +1911,9 @@ collect_select_kernel(MfmaArgs p) {
     const int wave = threadIdx.x >> 6;
     const int local = blockIdx.x * 4 + wave;
     if (local >= p.rows) return;  // whole waves; no block barrier below
+    // staged: the record says "not emitted" until the emit overwrites it
+    if (p.stage && lane == 0) p.stage[static_cast<size_t>(local) * (2 * p.k + 2) + 2 * p.k + 1] = 0;
+    const int stage_row = p.stage ? local : -1;
     if (*p.guard) return;         // the sweep queued every row for the radix path
     if (p.heavy && p.heavy[local]) return;  // already queued for the list pass
     const int qi = query_of(p, local);
@@ -1936,16 +1974,35 @@ collect_select_kernel(MfmaArgs p) {
     }
     wave_lds_sync();
     if constexpr (WIDE) {
-        wide_select(p, s_words[wave], total, qi, lane);
+        wide_select(p, s_words[wave], total, qi, lane, stage_row);
     } else if (total <= 128) {
-        select_words<2>(p, s_words[wave], total, qi, lane, s_sel[wave]);
+        select_words<2>(p, s_words[wave], total, qi, lane, s_sel[wave], stage_row);
     } else if (total <= 256) {
-        select_words<4>(p, s_words[wave], total, qi, lane, s_sel[wave]);
+        select_words<4>(p, s_words[wave], total, qi, lane, s_sel[wave], stage_row);
     } else if (kWaveMax <= 512 || total <= 512) {
-        select_words<8>(p, s_words[wave], total, qi, lane, s_sel[wave]);
+        select_words<8>(p, s_words[wave], total, qi, lane, s_sel[wave], stage_row);
     } else if constexpr (kWaveMax > 512) {
-        select_words<16>(p, s_words[wave], total, qi, lane, s_sel[wave]);
+        select_words<16>(p, s_words[wave], total, qi, lane, s_sel[wave], stage_row);
     }
+}
+
+// The staged records of the first-pass wave select into the reference's
+// transposed outputs: one thread per original row i (record of position
+// qpos[i]), so each slot's stores are consecutive words. Rows the select did
+// not emit (flag 0) belong to the later passes and are left alone.
+__global__ void __launch_bounds__(256)
+unstage_kernel(const int* __restrict__ stage, const int* __restrict__ qpos, int n, int k,
+               int* __restrict__ nbr_idx, float* __restrict__ nbr_d2, float* __restrict__ kdist2) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int* rec = stage + static_cast<size_t>(qpos[i]) * (2 * k + 2);
+    if (rec[2 * k + 1] == 0) return;
+    const size_t ns = static_cast<size_t>(n);
+    for (int e = 0; e < k; ++e) {
+        nbr_idx[static_cast<size_t>(e) * ns + i] = rec[e];
+        nbr_d2[static_cast<size_t>(e) * ns + i] = __int_as_float(rec[k + e]);
+    }
+    kdist2[i] = __int_as_float(rec[2 * k]);
 }
 
 // One block per queued row (grid-stride). The row's collected entries are
@@ -2330,6 +2387,10 @@ void launch_collect(const MfmaArgs& a, hipStream_t stream) {
             } else {
                 collect_select_kernel<DP, false><<<(a.rows + 3) / 4, 256, 0, stream>>>(a);
             }
+            if (a.stage) {
+                unstage_kernel<<<(a.n + 255) / 256, 256, 0, stream>>>(a.stage, a.qpos, a.n, a.k, a.nbr_idx,
+                                                                      a.nbr_d2, a.kdist2);
+            }
         }
         ProfileScope prof("big_select", stream);
         if (wide) {
@@ -2411,6 +2472,10 @@ void launch_sym(const MfmaArgs& a, const SymBuffers& sb, hipStream_t stream) {
             collect_select_kernel<DP, true><<<(a.rows + 3) / 4, 256, 0, stream>>>(a);
         } else {
             collect_select_kernel<DP, false><<<(a.rows + 3) / 4, 256, 0, stream>>>(a);
+        }
+        if (a.stage) {
+            unstage_kernel<<<(a.n + 255) / 256, 256, 0, stream>>>(a.stage, a.qpos, a.n, a.k, a.nbr_idx, a.nbr_d2,
+                                                                  a.kdist2);
         }
     }
     ProfileScope prof("big_select", stream);
@@ -2675,6 +2740,8 @@ size_t mfma_scratch_bytes(int n_points, int dp, int form, int rows, int k, bool 
                 second * kSecondCap * sizeof(uint32_t) + second * sizeof(int) +
                 static_cast<size_t>(std::min(rows, kListRows)) * (sizeof(int) +
                     (k <= kListKMax && dp < kWideMin ? sizeof(uint64_t) * second_slices(n_points) * k : 0)) + 16 * 256;
+        // the wave select's staged records (a full run in candidate order)
+        lists += static_cast<size_t>(rows) * (2 * k + 2) * sizeof(int) + 256;
         if (k > kListKMax || dp >= kWideMin) {  // the list rows' stored keys
             lists += static_cast<size_t>(brute_batch_rows(n_points, std::min(rows, kListRows))) * n_points *
                      sizeof(float) + 256;
@@ -2763,6 +2830,11 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
             args.ccount = static_cast<int*>(take(slots * sizeof(int)));
         }
         args.big_rows = static_cast<int*>(take(static_cast<size_t>(rows) * sizeof(int)));
+        // A full run in candidate order stages its first-pass lists by
+        // position and writes the transposed outputs in original order.
+        if (ord && qlist == ord->order && row0 == 0 && rows == n_points) {
+            args.stage = static_cast<int*>(take(static_cast<size_t>(rows) * (2 * k + 2) * sizeof(int)));
+        }
         args.ov_cap = std::min(rows, kSecondRows);
         args.ov_rows = static_cast<int*>(take(static_cast<size_t>(args.ov_cap) * sizeof(int)));
         int* counters = static_cast<int*>(take(4 * sizeof(int)));
@@ -2783,6 +2855,7 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
         second.big_rows = nullptr;
         second.big_count = nullptr;
         second.heavy = nullptr;
+        second.stage = nullptr;
         // Its overflow queue feeds the list pass.
         second.ov_cap = std::min(rows, kListRows);
         second.ov_rows = static_cast<int*>(take(static_cast<size_t>(second.ov_cap) * sizeof(int)));
@@ -2800,6 +2873,7 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
         list.cbuf = nullptr;
         list.ccount = nullptr;
         list.heavy = nullptr;
+        list.stage = nullptr;
         if (k > kListKMax || dp >= kWideMin) {
             brute_batch = brute_batch_rows(n_points, list.rows);
             brute_keys = static_cast<float*>(take(static_cast<size_t>(brute_batch) * n_points * sizeof(float)));
