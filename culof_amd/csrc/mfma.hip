@@ -390,6 +390,10 @@ struct MfmaArgs {
     int* stage;
 };
 
+// Words per staged record: k indices, k keys, the k-th key and the emitted
+// flag, padded to whole 16-byte groups.
+__host__ __device__ constexpr int stage_words(int k) { return (2 * k + 2 + 3) & ~3; }
+
 // Streams candidate tiles [t_begin, t_end) in chunks of CT tiles through a
 // double-buffered LDS stage shared by the block's 4 waves: while the waves
 // screen chunk c out of one buffer, every thread holds its share of chunk
@@ -1612,7 +1616,7 @@ __device__ void emit_selected(const MfmaArgs& p, const uint64_t* sel, int qi, in
         }
     }
     if (local >= 0) {  // staged: one contiguous record per row
-        int* rec = p.stage + static_cast<size_t>(local) * (2 * k + 2);
+        int* rec = p.stage + static_cast<size_t>(local) * stage_words(k);
         if (valid) {
             rec[lane] = static_cast<int>(static_cast<uint32_t>(s));
             rec[k + lane] = static_cast<int>(key);
@@ -1772,7 +1776,7 @@ __device__ __forceinline__ void emit_selected_wide(const MfmaArgs& p, const uint
         }
     }
     if (local >= 0) {  // staged record (see emit_selected)
-        int* rec = p.stage + static_cast<size_t>(local) * (2 * k + 2);
+        int* rec = p.stage + static_cast<size_t>(local) * stage_words(k);
 #pragma unroll
         for (int u = 0; u < KW; ++u) {
             const int e = 64 * u + lane;
@@ -1912,7 +1916,7 @@ collect_select_kernel(MfmaArgs p) {
     const int local = blockIdx.x * 4 + wave;
     if (local >= p.rows) return;  // whole waves; no block barrier below
     // staged: the record says "not emitted" until the emit overwrites it
-    if (p.stage && lane == 0) p.stage[static_cast<size_t>(local) * (2 * p.k + 2) + 2 * p.k + 1] = 0;
+    if (p.stage && lane == 0) p.stage[static_cast<size_t>(local) * stage_words(p.k) + 2 * p.k + 1] = 0;
     const int stage_row = p.stage ? local : -1;
     if (*p.guard) return;         // the sweep queued every row for the radix path
     if (p.heavy && p.heavy[local]) return;  // already queued for the list pass
@@ -1990,19 +1994,38 @@ collect_select_kernel(MfmaArgs p) {
 // transposed outputs: one thread per original row i (record of position
 // qpos[i]), so each slot's stores are consecutive words. Rows the select did
 // not emit (flag 0) belong to the later passes and are left alone.
+// The record is read as 16-byte groups, eight at a time, so a wave's 64
+// records are consumed while their cache lines are still resident.
 __global__ void __launch_bounds__(256)
 unstage_kernel(const int* __restrict__ stage, const int* __restrict__ qpos, int n, int k,
                int* __restrict__ nbr_idx, float* __restrict__ nbr_d2, float* __restrict__ kdist2) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const int* rec = stage + static_cast<size_t>(qpos[i]) * (2 * k + 2);
-    if (rec[2 * k + 1] == 0) return;
+    const int words = stage_words(k);
+    const int4* rec = reinterpret_cast<const int4*>(stage + static_cast<size_t>(qpos[i]) * words);
+    const int flag_at = 2 * k + 1;
+    if (reinterpret_cast<const int*>(rec)[flag_at] == 0) return;
     const size_t ns = static_cast<size_t>(n);
-    for (int e = 0; e < k; ++e) {
-        nbr_idx[static_cast<size_t>(e) * ns + i] = rec[e];
-        nbr_d2[static_cast<size_t>(e) * ns + i] = __int_as_float(rec[k + e]);
+    for (int g0 = 0; g0 < words / 4; g0 += 8) {
+        int4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = g0 + u < words / 4 ? rec[g0 + u] : make_int4(0, 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int w[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int e = 4 * (g0 + u) + c;
+                if (e < k) {
+                    nbr_idx[static_cast<size_t>(e) * ns + i] = w[c];
+                } else if (e < 2 * k) {
+                    nbr_d2[static_cast<size_t>(e - k) * ns + i] = __int_as_float(w[c]);
+                } else if (e == 2 * k) {
+                    kdist2[i] = __int_as_float(w[c]);
+                }
+            }
+        }
     }
-    kdist2[i] = __int_as_float(rec[2 * k]);
 }
 
 // One block per queued row (grid-stride). The row's collected entries are
@@ -2741,7 +2764,7 @@ size_t mfma_scratch_bytes(int n_points, int dp, int form, int rows, int k, bool 
                 static_cast<size_t>(std::min(rows, kListRows)) * (sizeof(int) +
                     (k <= kListKMax && dp < kWideMin ? sizeof(uint64_t) * second_slices(n_points) * k : 0)) + 16 * 256;
         // the wave select's staged records (a full run in candidate order)
-        lists += static_cast<size_t>(rows) * (2 * k + 2) * sizeof(int) + 256;
+        lists += static_cast<size_t>(rows) * stage_words(k) * sizeof(int) + 256;
         if (k > kListKMax || dp >= kWideMin) {  // the list rows' stored keys
             lists += static_cast<size_t>(brute_batch_rows(n_points, std::min(rows, kListRows))) * n_points *
                      sizeof(float) + 256;
@@ -2833,7 +2856,7 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
         // A full run in candidate order stages its first-pass lists by
         // position and writes the transposed outputs in original order.
         if (ord && qlist == ord->order && row0 == 0 && rows == n_points) {
-            args.stage = static_cast<int*>(take(static_cast<size_t>(rows) * (2 * k + 2) * sizeof(int)));
+            args.stage = static_cast<int*>(take(static_cast<size_t>(rows) * stage_words(k) * sizeof(int)));
         }
         args.ov_cap = std::min(rows, kSecondRows);
         args.ov_rows = static_cast<int*>(take(static_cast<size_t>(args.ov_cap) * sizeof(int)));

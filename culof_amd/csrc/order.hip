@@ -407,12 +407,23 @@ seed_hist_kernel(const float* __restrict__ tseed, int n, int* __restrict__ hist)
 // histogram is staged in LDS, then one thread walks it down from the top.
 __global__ void __launch_bounds__(kBlock)
 seed_split_bin_kernel(const int* __restrict__ hist, int target, int* __restrict__ small) {
+    constexpr int kPer = kSeedBins / kBlock;  // bins per thread, consecutive
     __shared__ int s_hist[kSeedBins];
-    for (int b = threadIdx.x; b < kSeedBins; b += kBlock) s_hist[b] = hist[b];
+    __shared__ int s_part[kBlock];
+    int part = 0;
+    for (int e = 0; e < kPer; ++e) {
+        const int b = threadIdx.x * kPer + e;
+        s_hist[b] = hist[b];
+        part += s_hist[b];
+    }
+    s_part[threadIdx.x] = part;
     __syncthreads();
     if (threadIdx.x != 0) return;
-    int bin = kSeedBins;
+    // whole groups of kPer bins from the top while they fit, then single bins
+    int group = kBlock;
     int count = 0;
+    while (group > 0 && count + s_part[group - 1] <= target) count += s_part[--group];
+    int bin = group * kPer;
     while (bin > 0 && count + s_hist[bin - 1] <= target) count += s_hist[--bin];
     small[0] = bin;
     small[1] = count;
