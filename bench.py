@@ -359,7 +359,9 @@ def main() -> int:
     # ---- roofline of the dominant kernel (the symmetric collect pass) ----
     avg_sweep_s = sweep_ms / max(sweep_launches, 1) / 1e3
     dp = -(-d // 4) * 4 if d > 4 else 4
-    all_pairs_flops = executed_flops(n, dp, _native.FORM_PLAIN)
+    # (rows past 128 features take the K-chunked pass over every ordered pair)
+    all_pairs_flops = (executed_flops(n, dp, _native.FORM_PLAIN) if d <= 128 else
+                       2.0 * (1 << (d - 1).bit_length()) * float(-(-n // 32) * 32) ** 2)
     # the pass screens only the tile pairs its pruning keeps (counted on the device)
     executed = (tile_pairs / max(sweep_launches, 1) * 32.0 * 32.0 * 2.0 * dp) if tile_pairs else all_pairs_flops
     achieved = executed / avg_sweep_s / 1e12

@@ -95,28 +95,58 @@ column_finalize_kernel(const double* __restrict__ part, int n, int d, int pass,
 // One thread per row: centred float32 row (zero-padded to `stride`), its
 // norm, and optionally a zero-padded copy of the input row (the key rows when
 // not normalising). Raises *guard when a norm is too large for the Gram
-// screen to stay finite (above FLT_MAX / 16, or not finite).
-__global__ void __launch_bounds__(256)
+// screen to stay finite (above FLT_MAX / 16, or not finite). The block's 256
+// rows pass through LDS 16 features at a time, so the loads and stores of
+// consecutive threads are consecutive features of a row; each thread still
+// sums its row's squares in ascending feature order.
+constexpr int kApplyRows = 256;
+constexpr int kApplyChunk = 16;
+
+__global__ void __launch_bounds__(kApplyRows)
 center_apply_kernel(const float* __restrict__ x, int n, int d, const double* __restrict__ mean,
                     const double* __restrict__ inv_std, float* __restrict__ out, int stride,
                     float* __restrict__ norms, float* __restrict__ copy, int* __restrict__ guard) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float* row = x + static_cast<size_t>(i) * d;
-    float* dst = out + static_cast<size_t>(i) * stride;
-    float* cpy = copy ? copy + static_cast<size_t>(i) * stride : nullptr;
+    __shared__ float s_v[kApplyRows][kApplyChunk + 1];
+    const int tid = threadIdx.x;
+    const int r0 = blockIdx.x * kApplyRows;
     double s = 0.0;
-    for (int t = 0; t < d; ++t) {
-        const float xv = row[t];
-        const float v = static_cast<float>((static_cast<double>(xv) - mean[t]) * inv_std[t]);
-        dst[t] = v;
-        if (cpy) cpy[t] = xv;
-        s += static_cast<double>(v) * static_cast<double>(v);  // exact product
+    for (int f0 = 0; f0 < stride; f0 += kApplyChunk) {
+        // load: element e of the chunk is row e / kApplyChunk, feature e % kApplyChunk
+#pragma unroll
+        for (int u = 0; u < kApplyChunk; ++u) {
+            const int e = tid + kApplyRows * u;
+            const int r = e / kApplyChunk;
+            const int f = f0 + (e - r * kApplyChunk);
+            const int i = r0 + r;
+            float xv = 0.0f;
+            if (i < n && f < d) xv = x[static_cast<size_t>(i) * d + f];
+            if (copy && i < n && f < stride) copy[static_cast<size_t>(i) * stride + f] = xv;
+            s_v[r][e - r * kApplyChunk] = xv;
+        }
+        __syncthreads();
+        // this thread's row, features in ascending order
+        for (int c = 0; c < kApplyChunk; ++c) {
+            const int f = f0 + c;
+            float v = 0.0f;
+            if (f < d) {
+                v = static_cast<float>((static_cast<double>(s_v[tid][c]) - mean[f]) * inv_std[f]);
+                s += static_cast<double>(v) * static_cast<double>(v);  // exact product
+            }
+            s_v[tid][c] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kApplyChunk; ++u) {
+            const int e = tid + kApplyRows * u;
+            const int r = e / kApplyChunk;
+            const int f = f0 + (e - r * kApplyChunk);
+            const int i = r0 + r;
+            if (i < n && f < stride) out[static_cast<size_t>(i) * stride + f] = s_v[r][e - r * kApplyChunk];
+        }
+        __syncthreads();
     }
-    for (int t = d; t < stride; ++t) {
-        dst[t] = 0.0f;
-        if (cpy) cpy[t] = 0.0f;
-    }
+    const int i = r0 + tid;
+    if (i >= n) return;
     const float nf = static_cast<float>(s);
     norms[i] = nf;
     if (guard && !(nf <= 2.1267647e37f)) atomicOr(guard, 1);  // FLT_MAX / 16, NaN
@@ -174,7 +204,7 @@ void center_and_norms(const float* points, int n_points, int n_dims, bool scale,
     }
     {
         ProfileScope prof("center_apply", stream);
-        center_apply_kernel<<<(n_points + 255) / 256, 256, 0, stream>>>(
+        center_apply_kernel<<<(n_points + kApplyRows - 1) / kApplyRows, kApplyRows, 0, stream>>>(
             points, n_points, n_dims, mean, inv_std, centred, stride, norms, copy, guard);
         CULOF_CHECK(hipGetLastError());
     }
@@ -185,8 +215,8 @@ void apply_rows(const float* rows, int n_rows, int n_dims, const double* work, f
     const double* mean = work + static_cast<size_t>(kLanes) * n_dims;
     const double* inv_std = mean + n_dims;
     if (n_rows <= 0) return;
-    center_apply_kernel<<<(n_rows + 255) / 256, 256, 0, stream>>>(rows, n_rows, n_dims, mean, inv_std, out,
-                                                                  stride, norms, copy, nullptr);
+    center_apply_kernel<<<(n_rows + kApplyRows - 1) / kApplyRows, kApplyRows, 0, stream>>>(
+        rows, n_rows, n_dims, mean, inv_std, out, stride, norms, copy, nullptr);
     CULOF_CHECK(hipGetLastError());
 }
 

@@ -93,6 +93,9 @@ WORKLOADS = {
     "bf16-100k": dict(gen="lowrank", n=100_000, d=128, k=50),
     "grad-50k": dict(gen="grad", n=50_000, d=32, k=20),
     "ties-500k": dict(gen="ties", n=500_000, d=64, k=32),
+    # shapes past the list kernels (k > 64, d > 128)
+    "k128-200k": dict(gen="clusters", n=200_000, d=8, k=128),
+    "d256-200k": dict(gen="lowrank", n=200_000, d=256, k=20),
 }
 
 
