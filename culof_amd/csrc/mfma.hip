@@ -1302,6 +1302,56 @@ tile_bounds_kernel(const float* __restrict__ xs_ord, const int* __restrict__ ord
     if (f == 0) tmax[t] = tm;
 }
 
+// Bounds of groups of kPruneGroup consecutive tiles (positions), for the
+// first test of sym_live_chunks_kernel.
+constexpr int kPruneGroup = 8;
+
+template <int DP>
+__global__ void __launch_bounds__(256)
+group_bounds_kernel(const float* __restrict__ box, const float* __restrict__ tmax, int tiles,
+                    float* __restrict__ gbox, float* __restrict__ gtmax) {
+    const int groups = (tiles + kPruneGroup - 1) / kPruneGroup;
+    const int e = blockIdx.x * 256 + threadIdx.x;  // (group, feature)
+    if (e >= groups * DP) return;
+    const int g = e / DP;
+    const int f = e - g * DP;
+    float lo = __uint_as_float(kInfBits);
+    float hi = -__uint_as_float(kInfBits);
+    float tm = -__uint_as_float(kInfBits);
+    for (int t = g * kPruneGroup; t < min(tiles, (g + 1) * kPruneGroup); ++t) {
+        lo = __builtin_fminf(lo, box[(static_cast<size_t>(t) * 2) * DP + f]);
+        hi = __builtin_fmaxf(hi, box[(static_cast<size_t>(t) * 2 + 1) * DP + f]);
+        if (f == 0) tm = __builtin_fmaxf(tm, tmax[t]);
+    }
+    gbox[(static_cast<size_t>(g) * 2) * DP + f] = lo;
+    gbox[(static_cast<size_t>(g) * 2 + 1) * DP + f] = hi;
+    if (f == 0) gtmax[g] = tm;
+}
+
+// Squared gap between two boxes (lo / hi rows of DP floats; the second in LDS).
+template <int DP>
+__device__ __forceinline__ float box_gap2(const float* __restrict__ lo, const float* __restrict__ hi,
+                                          const float* blo, const float* bhi) {
+    const float4* lo4 = reinterpret_cast<const float4*>(lo);
+    const float4* hi4 = reinterpret_cast<const float4*>(hi);
+    const float4* blo4 = reinterpret_cast<const float4*>(blo);
+    const float4* bhi4 = reinterpret_cast<const float4*>(bhi);
+    float lb = 0.0f;
+#pragma unroll 8
+    for (int f4 = 0; f4 < DP / 4; ++f4) {
+        const float4 l = lo4[f4], hh = hi4[f4], bl = blo4[f4], bh = bhi4[f4];
+        float g = __builtin_fmaxf(0.0f, __builtin_fmaxf(l.x - bh.x, bl.x - hh.x));
+        lb = __builtin_fmaf(g, g, lb);
+        g = __builtin_fmaxf(0.0f, __builtin_fmaxf(l.y - bh.y, bl.y - hh.y));
+        lb = __builtin_fmaf(g, g, lb);
+        g = __builtin_fmaxf(0.0f, __builtin_fmaxf(l.z - bh.z, bl.z - hh.z));
+        lb = __builtin_fmaf(g, g, lb);
+        g = __builtin_fmaxf(0.0f, __builtin_fmaxf(l.w - bh.w, bl.w - hh.w));
+        lb = __builtin_fmaf(g, g, lb);
+    }
+    return lb;
+}
+
 // For block B and slice y of the symmetric pass: the starts of its candidate
 // chunks (CT tiles from kBlockTiles (B + o0)) that may hold a pair, in
 // ascending order, and their count. A chunk stays when one of its tiles
@@ -1310,7 +1360,8 @@ tile_bounds_kernel(const float* __restrict__ xs_ord, const int* __restrict__ ord
 // round). The block's own tiles (lower bound 0) always stay.
 template <int DP, int CT, int kBlockTiles>
 __global__ void __launch_bounds__(256)
-sym_live_chunks_kernel(const float* __restrict__ box, const float* __restrict__ tmax, int nb, int ns,
+sym_live_chunks_kernel(const float* __restrict__ box, const float* __restrict__ tmax,
+                       const float* __restrict__ gbox, const float* __restrict__ gtmax, int nb, int ns,
                        int stride, int* __restrict__ live, int* __restrict__ live_count,
                        unsigned long long* __restrict__ visited) {
     static_assert(256 % CT == 0, "whole chunks per round");
@@ -1359,25 +1410,16 @@ sym_live_chunks_kernel(const float* __restrict__ box, const float* __restrict__ 
         int ok = 0;
         if (v < t_end) {
             const size_t pt = static_cast<size_t>(v >= wrap ? v - wrap : v);
-            const float4* lo4 = reinterpret_cast<const float4*>(box + (pt * 2) * DP);
-            const float4* hi4 = reinterpret_cast<const float4*>(box + (pt * 2 + 1) * DP);
-            const float4* blo4 = reinterpret_cast<const float4*>(s_lo);
-            const float4* bhi4 = reinterpret_cast<const float4*>(s_hi);
-            float lb = 0.0f;
-#pragma unroll 8
-            for (int f4 = 0; f4 < DP / 4; ++f4) {
-                const float4 l = lo4[f4], hh = hi4[f4], bl = blo4[f4], bh = bhi4[f4];
-                float g = __builtin_fmaxf(0.0f, __builtin_fmaxf(l.x - bh.x, bl.x - hh.x));
-                lb = __builtin_fmaf(g, g, lb);
-                g = __builtin_fmaxf(0.0f, __builtin_fmaxf(l.y - bh.y, bl.y - hh.y));
-                lb = __builtin_fmaf(g, g, lb);
-                g = __builtin_fmaxf(0.0f, __builtin_fmaxf(l.z - bh.z, bl.z - hh.z));
-                lb = __builtin_fmaf(g, g, lb);
-                g = __builtin_fmaxf(0.0f, __builtin_fmaxf(l.w - bh.w, bl.w - hh.w));
-                lb = __builtin_fmaf(g, g, lb);
+            // the tile's group first (its box holds the tile's, its seed
+            // bound is at least the tile's): most far tiles fall away there,
+            // and the group's bounds are the same loads for its 8 threads
+            const size_t pg = pt / kPruneGroup;
+            const float lbg = box_gap2<DP>(gbox + (pg * 2) * DP, gbox + (pg * 2 + 1) * DP, s_lo, s_hi);
+            if (!(lbg * shrink > __builtin_fmaxf(s_tq, gtmax[pg]))) {
+                const float lb = box_gap2<DP>(box + (pt * 2) * DP, box + (pt * 2 + 1) * DP, s_lo, s_hi);
+                const float thr = __builtin_fmaxf(s_tq, tmax[pt]);
+                ok = !(lb * shrink > thr);  // NaN keeps
             }
-            const float thr = __builtin_fmaxf(s_tq, tmax[pt]);
-            ok = !(lb * shrink > thr);  // NaN keeps
         }
         s_ok[tid] = ok;
         __syncthreads();
@@ -2446,6 +2488,8 @@ struct SymBuffers {
     // pruning: per-tile boxes and largest seeds, per (block, slice) the live chunks
     float* box = nullptr;
     float* tmax = nullptr;
+    float* gbox = nullptr;   // per group of kPruneGroup tiles
+    float* gtmax = nullptr;
     int* live = nullptr;
     int* live_count = nullptr;
     int live_stride = 0;
@@ -2463,9 +2507,13 @@ void launch_sym(const MfmaArgs& a, const SymBuffers& sb, hipStream_t stream) {
         const int tiles = kBlockTiles * nb;
         tile_bounds_kernel<DP><<<(tiles * DP + 255) / 256, 256, 0, stream>>>(
             a.xs_cand, a.cand, a.tseed, a.heavy, a.n, tiles, sb.box, sb.tmax);
+        const int groups = (tiles + kPruneGroup - 1) / kPruneGroup;
+        group_bounds_kernel<DP><<<(groups * DP + 255) / 256, 256, 0, stream>>>(sb.box, sb.tmax, tiles, sb.gbox,
+                                                                              sb.gtmax);
         sym_live_chunks_kernel<DP, sym_chunk_tiles<Op, DP>(), kBlockTiles>
-            <<<dim3(nb, sb.slices), 256, 0, stream>>>(sb.box, sb.tmax, nb, sb.slices, sb.live_stride, sb.live,
-                                                      sb.live_count, profile_tile_counter());
+            <<<dim3(nb, sb.slices), 256, 0, stream>>>(sb.box, sb.tmax, sb.gbox, sb.gtmax, nb, sb.slices,
+                                                      sb.live_stride, sb.live, sb.live_count,
+                                                      profile_tile_counter());
         CULOF_CHECK(hipGetLastError());
     } else if (unsigned long long* counter = profile_tile_counter()) {
         unsigned long long pairs = 0;
@@ -2737,8 +2785,9 @@ PruneShape prune_shape(int n_points, int dp, int form, int ns) {
 
 size_t prune_bytes(int n_points, int dp, int form, int ns) {
     const PruneShape ps = prune_shape(n_points, dp, form, ns);
-    return static_cast<size_t>(ps.tiles) * (2 * dp + 1) * sizeof(float) +
-           static_cast<size_t>(ps.nb) * ns * (ps.stride + 1) * sizeof(int) + 4 * 256;
+    const size_t groups = (static_cast<size_t>(ps.tiles) + kPruneGroup - 1) / kPruneGroup;
+    return (static_cast<size_t>(ps.tiles) + groups) * (2 * dp + 1) * sizeof(float) +
+           static_cast<size_t>(ps.nb) * ns * (ps.stride + 1) * sizeof(int) + 6 * 256;
 }
 
 size_t mfma_scratch_bytes(int n_points, int dp, int form, int rows, int k, bool collect) {
@@ -2839,6 +2888,9 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
                 const PruneShape ps = prune_shape(n_points, dp, form, sb.slices);
                 sb.box = static_cast<float*>(take(static_cast<size_t>(ps.tiles) * 2 * dp * sizeof(float)));
                 sb.tmax = static_cast<float*>(take(static_cast<size_t>(ps.tiles) * sizeof(float)));
+                const size_t groups = (static_cast<size_t>(ps.tiles) + kPruneGroup - 1) / kPruneGroup;
+                sb.gbox = static_cast<float*>(take(groups * 2 * dp * sizeof(float)));
+                sb.gtmax = static_cast<float*>(take(groups * sizeof(float)));
                 sb.live_stride = ps.stride;
                 sb.live = static_cast<int*>(take(static_cast<size_t>(ps.nb) * sb.slices * ps.stride * sizeof(int)));
                 sb.live_count = static_cast<int*>(take(static_cast<size_t>(ps.nb) * sb.slices * sizeof(int)));
