@@ -69,7 +69,7 @@ KERNELS = ("center_partials", "center_apply", "order", "seed", "order_split", "h
            "second_select", "mfma_sweep", "merge_slices", "sweep_knn", "brute_rows", "radix_knn", "tie_break",
            "tie_collect", "tie_band", "lrd", "score")
 # Timing scope -> kernel name in the rocprofv3 counter summary.
-HBM_KERNELS = {"collect_select": "collect_select_kernel<8>", "scatter_pairs": "scatter_pairs_kernel"}
+HBM_KERNELS = {"collect_select": "collect_select_kernel<8, false>", "scatter_pairs": "scatter_pairs_kernel"}
 
 
 def log(msg: str) -> None:
