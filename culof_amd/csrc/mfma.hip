@@ -238,6 +238,47 @@ struct ScreenBF16 {
             acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, b1.hi[m], acc1, 0, 0, 0);
         }
     }
+    // The two phases of run2 for a screen in two levels: the hi x hi
+    // products alone (a coarse value within coarse_slack of the full one),
+    // then the lo x hi and hi x lo corrections, for the tiles that need them.
+    // hi(a) hi(b) + lo(a) hi(b) + hi(a) lo(b) are the products of run2; only
+    // their order of accumulation differs, which the slack below does not
+    // depend on.
+    template <int DP>
+    __device__ static void run2_hi(const Elem* half_row, const Frag<DP>& b0, const Frag<DP>& b1, floatx16& acc0,
+                                   floatx16& acc1) {
+        const bf16x8* p = reinterpret_cast<const bf16x8*>(half_row);
+#pragma unroll
+        for (int m = 0; m < DP / 16; ++m) {
+            const bf16x8 ahi = p[2 * m];
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, b0.hi[m], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, b1.hi[m], acc1, 0, 0, 0);
+        }
+    }
+    template <int DP>
+    __device__ static void run2_lo(const Elem* half_row, const Frag<DP>& b0, const Frag<DP>& b1, floatx16& acc0,
+                                   floatx16& acc1) {
+        const bf16x8* p = reinterpret_cast<const bf16x8*>(half_row);
+#pragma unroll
+        for (int m = 0; m < DP / 16; ++m) {
+            const bf16x8 ahi = p[2 * m];
+            const bf16x8 alo = p[DP / 8 + 2 * m];
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, b0.hi[m], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, b1.hi[m], acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, b0.lo[m], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, b1.lo[m], acc1, 0, 0, 0);
+        }
+    }
+    // Coarse slack: |full - coarse| <= coarse_slack(DP) (N_q + N_c). The
+    // corrections are products with a bf16 residual, |lo(x)| <= 2^-8 |x|, so
+    // together at most 2^-7 (1 + 2^-8) sum |a_t b_t| <= 2^-7 (1 + 2^-8)
+    // (N_q + N_c) (b = -2 x_q, so sum |a_t b_t| <= N_c + N_q); their 2 DP
+    // additions to an accumulator of magnitude at most 3.1 (N_q + N_c) add
+    // 2 DP x 2u x 3.1 (counted at 2 ulp per addition, as above). The factor
+    // 1.01 and 16u leave room for the roundings of the widened thresholds.
+    __host__ __device__ static constexpr float coarse_slack(int dp) {
+        return 1.01f * (0.0078430176f + 12.4f * dp * kU) + 16.0f * kU;
+    }
     // 3.1 2^-16 for the split products, FP32 accumulation of 3 DP terms whose
     // magnitudes sum to at most 2 (N_q + N_i) (counted at 2 ulp per addition,
     // in case the matrix core truncates), and the norm, centring and
@@ -406,11 +447,19 @@ __host__ __device__ constexpr int stage_words(int k) { return (2 * k + 2 + 3) & 
 // column thresholds (8 float4s: rows 4i..4i+3). The operand arrays are padded
 // to whole tiles (zero rows, NaN norms and thresholds); tiles past t_end are
 // never visited. `stage` holds 2 * ChunkShape::kBytes (kSymBytes with kSym).
-template <class Op, int DP, bool kSym, int QT = 1, class Visit>
+//
+// Two query tiles per wave (QT = 2, the BF16 screen) screen in two levels:
+// coarse(acc, v, wmax, nmax, t) gets each query tile's accumulator after the
+// hi x hi products alone, with this half's maximum column threshold and
+// maximum adjusted norm over the tile (nmax, staged like wmax), and returns
+// whether the tile can hold a pair that passes; only then are the correction
+// products added and the tile visited.
+template <class Op, int DP, bool kSym, int QT = 1, class Visit, class Coarse>
 __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename Op::template Frag<DP>* bq,
                                              int j, int h, unsigned char* stage, int t_begin, int t_end,
                                              int wrap, const float* __restrict__ wcol,
                                              const float2* __restrict__ wmax, Visit&& visit,
+                                             const float2* __restrict__ nmax, Coarse&& coarse,
                                              const int* __restrict__ clist = nullptr, int nlist = 0) {
     static_assert(QT == 1 || (QT == 2 && kSym), "two query tiles per wave: symmetric pass only");
     using Frag = typename Op::template Frag<DP>;
@@ -419,6 +468,10 @@ __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename O
     // Two query tiles per wave: the A tiles go global -> LDS by DMA, two per
     // chunk (no staging registers).
     constexpr bool kDma = QT == 2;
+    // The two-level screen pays where it skips most of the corrections: at
+    // 128 features boxes hardly separate tiles, so the pruning keeps far pairs
+    // that the coarse test drops; at 64 most kept pairs are near ones.
+    constexpr bool kTwoLevel = QT == 2 && DP >= 128;
     constexpr int CT = kDma ? Shape::kDmaTiles : Shape::kTiles;
     constexpr int kRowsPerChunk = 32 * CT;
     constexpr int kRowF4 = Shape::kRowF4;     // float4s per global row (DP * elem >= 16 bytes)
@@ -435,6 +488,7 @@ __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename O
     float4* s_m = s_a + 2 * kLdsChunkF4;
     float4* s_w = s_m + 2 * kChunkM4;              // kSym: column thresholds
     float2* s_x = reinterpret_cast<float2*>(s_w + 2 * kChunkM4);  // kSym: per-tile maxima
+    float2* s_nx = s_x + 2 * CT;                   // QT = 2: per-tile maximum adjusted norms
 
     // Chunks of CT tiles from t_begin, or (clist) only the listed chunk
     // starts, nlist of them (the pruned symmetric pass).
@@ -487,9 +541,13 @@ __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename O
                     __builtin_amdgcn_global_load_lds(gw + at, s_w + dbuf * kChunkM4, 16, 0, 0);
                 }
                 const int vx = t0 + tid / 2;
-                if (tid < 2 * CT && vx < t_end)
+                if (tid < 2 * CT && vx < t_end) {
                     __builtin_amdgcn_global_load_lds(reinterpret_cast<const float*>(wmax + phys(vx)) + (tid & 1),
                                                      s_x + dbuf * CT, 4, 0, 0);
+                    if constexpr (kTwoLevel)
+                        __builtin_amdgcn_global_load_lds(reinterpret_cast<const float*>(nmax + phys(vx)) + (tid & 1),
+                                                         s_nx + dbuf * CT, 4, 0, 0);
+                }
             }
         } else {
 #pragma unroll
@@ -545,10 +603,20 @@ __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename O
         }
         return acc;
     };
-    float wm[kSym ? CT : 1];  // kSym: this half's column maxima of the chunk's tiles
+    // kSym: this half's column maxima of the chunk's tiles (the two-level
+    // screen reads them from LDS where used, to spare registers)
+    float wm[kSym && !kTwoLevel ? CT : 1];
+    auto wmax_at = [&](int buf, int tt) {
+        if constexpr (kTwoLevel) {
+            const float2 x = s_x[buf * CT + tt];
+            return h ? x.y : x.x;
+        } else {
+            return wm[tt];
+        }
+    };
     auto call = [&](const floatx16& acc, int buf, int tt, int v, int t) {
         if constexpr (kSym) {
-            visit(acc, v, wm[tt], s_w + buf * kChunkM4 + tt * 8, t);
+            visit(acc, v, wmax_at(buf, tt), s_w + buf * kChunkM4 + tt * 8, t);
         } else {
             visit(acc, v);
         }
@@ -571,14 +639,38 @@ __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename O
         if (ch + 1 < n_chunks) fetch(chunk_start(ch + 1), buf ^ 1);  // next chunk in flight
         const float4* sa = s_a + buf * kLdsChunkF4;
         const float4* sm = s_m + buf * kChunkM4;
-        if constexpr (kSym) {
+        if constexpr (kSym && !kTwoLevel) {
 #pragma unroll
             for (int tt = 0; tt < CT; ++tt) {
                 const float2 x = s_x[buf * CT + tt];
                 wm[tt] = h ? x.y : x.x;
             }
         }
-        if constexpr (QT == 2) {
+        if constexpr (kTwoLevel) {
+            // One tile per step against two query fragments: two independent
+            // MFMA chains per A value read; the hi x hi products first, the
+            // corrections (for both query tiles) only where a coarse test
+            // passes.
+#pragma unroll
+            for (int tt = 0; tt < CT; ++tt) {
+                floatx16 acc0 = init_acc(sm, tt);
+                floatx16 acc1 = acc0;
+                const Elem* arow = reinterpret_cast<const Elem*>(sa + (tt * 32 + j) * kPitchF4) +
+                                   Op::half_offset(DP, h);
+                Op::template run2_hi<DP>(arow, bq[0], bq[1], acc0, acc1);
+                const int v = t0 + tt;
+                if (v < t_end) {
+                    const float2 nx = s_nx[buf * CT + tt];
+                    const float nmh = h ? nx.y : nx.x;
+                    const float wmh = wmax_at(buf, tt);
+                    if (coarse(acc0, v, wmh, nmh, 0) || coarse(acc1, v, wmh, nmh, 1)) {
+                        Op::template run2_lo<DP>(arow, bq[0], bq[1], acc0, acc1);
+                        call(acc0, buf, tt, v, 0);
+                        call(acc1, buf, tt, v, 1);
+                    }
+                }
+            }
+        } else if constexpr (QT == 2) {
             // One tile per step against two query fragments: two independent
             // MFMA chains per A value read.
 #pragma unroll
@@ -628,7 +720,8 @@ __device__ __forceinline__ void stream_tiles(const MfmaArgs& p, const typename O
     const int n_tiles = (p.n + 31) / 32;
     const int t_begin = static_cast<int>((static_cast<long long>(n_tiles) * blockIdx.y) / p.slices);
     const int t_end = static_cast<int>((static_cast<long long>(n_tiles) * (blockIdx.y + 1)) / p.slices);
-    stream_range<Op, DP, false>(p, &bq, j, h, stage, t_begin, t_end, n_tiles, nullptr, nullptr, visit);
+    stream_range<Op, DP, false>(p, &bq, j, h, stage, t_begin, t_end, n_tiles, nullptr, nullptr, visit, nullptr,
+                                [](const floatx16&, int, float, float, int) { return true; });
 }
 
 // Wide lists need the whole register file: let those instantiations run at
@@ -1116,7 +1209,7 @@ template <class Op, int DP>
 __global__ void __launch_bounds__(64 * kTilesPerBlock)
 __attribute__((amdgpu_waves_per_eu(DP >= 128 || sym_query_tiles<Op, DP>() == 2 ? 2 : 4, 8)))
 mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2* __restrict__ wmax,
-                        uint64_t* __restrict__ pool, int* __restrict__ chunk_region,
+                        const float2* __restrict__ nmax, uint64_t* __restrict__ pool, int* __restrict__ chunk_region,
                         int* __restrict__ chunk_count, int* __restrict__ next_chunk, int max_chunks,
                         int* __restrict__ col_over, const int* __restrict__ live_chunks,
                         const int* __restrict__ live_count, int live_stride, int n_slices) {
@@ -1159,6 +1252,15 @@ mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2
 
     Frag bq[QT];
     float Tn[QT], Lq[QT];
+    // Two-level screen (QT = 2): the coarse value is within g (N_q + N_c) of
+    // the full one, so a tile can hold a passing pair only where
+    // min(coarse) <= Tn + g (N_q + N_c) (row test) or
+    // min(coarse) - Wmax <= L_q + g (N_q + N_c) plus the rounding of the
+    // column difference (column test); N_c <= nmax / (1 - c) and the 1.01 in
+    // g covers that factor. gq holds the per-query part of the margin (with
+    // 8u of |Tn| and |L_q| for the roundings of the widened thresholds).
+    float gq[QT];
+    constexpr float g = QT == 2 && DP >= 128 ? ScreenBF16::coarse_slack(DP) : 0.0f;
     const float c = p.c;
 #pragma unroll
     for (int t = 0; t < QT; ++t) {
@@ -1167,9 +1269,11 @@ mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2
         const float tq = p.tseed[qi[t]];
         // A heavy row collects nothing for itself (the list pass serves it)
         // but still serves as a candidate of the other rows.
-        Tn[t] = valid[t] && !p.heavy[q[t]] ? (__builtin_fmaf(tq, 1.0f + c, c * nq) - nq) + 4.0f * kU * (tq + nq)
-                                           : -__uint_as_float(kInfBits);
+        const bool tests = valid[t] && !p.heavy[q[t]];
+        Tn[t] = tests ? (__builtin_fmaf(tq, 1.0f + c, c * nq) - nq) + 4.0f * kU * (tq + nq)
+                      : -__uint_as_float(kInfBits);
         Lq[t] = valid[t] ? 4.0f * kU * nq - p.madj[q[t]] : __uint_as_float(0x7FC00000u);
+        gq[t] = g * nq + 8.0f * kU * (__builtin_fabsf(Tn[t]) + __builtin_fabsf(Lq[t]));
     }
     const int h_b = (nb - 1) / 2 + ((nb % 2 == 0 && B < nb / 2) ? 1 : 0);
     const long long span = h_b + 1;  // offsets 0..h_b
@@ -1254,12 +1358,22 @@ mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2
             }
         }
     };
+    // Coarse test of the two-level screen (QT = 2), on the hi x hi value.
+    auto coarse = [&](const floatx16& acc, int v, float wm, float nm, int t) {
+        float lo = acc[0];
+#pragma unroll
+        for (int r = 1; r < 16; ++r) lo = __builtin_fminf(lo, acc[r]);  // NaN rows drop out
+        const float m = __builtin_fmaf(g, nm, gq[t]);
+        const bool rowp = lo <= Tn[t] + m;
+        const bool colp = v >= v_col && lo - wm <= __builtin_fmaf(8.0f * kU, __builtin_fabsf(wm), Lq[t] + m);
+        return __any(rowp || colp);
+    };
     // Only the chunks sym_live_chunks_kernel kept (when it ran).
     const int region = B * ns + ys;
     const int* clist = live_chunks ? live_chunks + static_cast<size_t>(region) * live_stride : nullptr;
     const int nlist = live_chunks ? live_count[region] : 0;
     stream_range<Op, DP, true, QT>(p, bq, j, h, smem, kBlockTiles * (B + o0), kBlockTiles * (B + o1), wrap,
-                                   wcol, wmax, visit, clist, nlist);
+                                   wcol, wmax, visit, nmax, coarse, clist, nlist);
 #pragma unroll
     for (int t = 0; t < QT; ++t) {
         if (valid[t] && h == 0) p.ccount[static_cast<size_t>(q[t]) * nslot + ys] = cnt[t];
@@ -1584,16 +1698,26 @@ column_thresholds_kernel(const float* __restrict__ tseed, const float* __restric
     wcol[i] = static_cast<double>(f) < w ? __uint_as_float(__float_as_uint(f) + 1u) : f;
 }
 
+// Per (tile, half): the largest column threshold and the largest adjusted
+// norm of the half's 16 rows (rows 8 b + 4 h + i, the rows one lane of the
+// screen holds).
 __global__ void __launch_bounds__(256)
-column_maxima_kernel(const float* __restrict__ wcol, int tiles, float2* __restrict__ wmax) {
+column_maxima_kernel(const float* __restrict__ wcol, const float* __restrict__ madj, int tiles,
+                     float2* __restrict__ wmax, float2* __restrict__ nmax) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= 2 * tiles) return;
     const int tile = e >> 1;
     const int h = e & 1;
     float m = __uint_as_float(0x7FC00000u);
-    for (int b = 0; b < 4; ++b)
-        for (int i = 0; i < 4; ++i) m = __builtin_fmaxf(m, wcol[tile * 32 + 8 * b + 4 * h + i]);
+    float a = __uint_as_float(0x7FC00000u);
+    for (int b = 0; b < 4; ++b) {
+        for (int i = 0; i < 4; ++i) {
+            m = __builtin_fmaxf(m, wcol[tile * 32 + 8 * b + 4 * h + i]);
+            a = __builtin_fmaxf(a, madj[tile * 32 + 8 * b + 4 * h + i]);
+        }
+    }
     reinterpret_cast<float*>(wmax)[e] = m;
+    reinterpret_cast<float*>(nmax)[e] = a;
 }
 
 // Collected entries per query the wave select holds; larger sets go to the
@@ -2478,6 +2602,7 @@ void launch_collect(const MfmaArgs& a, hipStream_t stream) {
 struct SymBuffers {
     float* wcol;
     float2* wmax;
+    float2* nmax;        // per (tile, half): largest adjusted norm (the two-level screen's margin)
     uint64_t* pool;      // column words, in chunks of kChunkWords
     int* chunk_region;   // per chunk: query tile * S + slice of the wave that owns it
     int* chunk_count;    // per chunk: words written
@@ -2526,7 +2651,7 @@ void launch_sym(const MfmaArgs& a, const SymBuffers& sb, hipStream_t stream) {
     {
         ProfileScope prof("mfma_collect", stream);
         mfma_sym_collect_kernel<Op, DP><<<nb * sb.slices, 64 * kTilesPerBlock, 0, stream>>>(
-            a, sb.wcol, sb.wmax, sb.pool, sb.chunk_region, sb.chunk_count, sb.next_chunk, sb.max_chunks,
+            a, sb.wcol, sb.wmax, sb.nmax, sb.pool, sb.chunk_region, sb.chunk_count, sb.next_chunk, sb.max_chunks,
             sb.col_over, sb.live, sb.live_count, sb.live_stride, sb.slices);
     }
     {
@@ -2802,7 +2927,7 @@ size_t mfma_scratch_bytes(int n_points, int dp, int form, int rows, int k, bool 
         const size_t second = static_cast<size_t>(std::min(rows, kSecondRows)) * second_slices(n_points);
         const size_t ss = static_cast<size_t>(sym_slices(n_points, sym_qt(dp, form)));
         const size_t sym = static_cast<size_t>(rows) * (2 * ss * (first_cap(k) + 1) + 2) * sizeof(uint32_t) +
-                           padded * sizeof(float) + padded / 32 * sizeof(float2) +
+                           padded * sizeof(float) + 2 * (padded / 32 * sizeof(float2) + 256) +
                            static_cast<size_t>(sym_chunks(n_points)) * (kChunkWords * sizeof(uint64_t) + 2 * sizeof(int)) +
                            prune_bytes(n_points, dp, form, static_cast<int>(ss)) + 2048;
         // (the symmetric pass serves full runs only)
@@ -2877,6 +3002,7 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
             args.heavy = heavy;
             sb.wcol = static_cast<float*>(take(sizeof(float) * padded_rows));
             sb.wmax = static_cast<float2*>(take(sizeof(float2) * (padded_rows / 32)));
+            sb.nmax = static_cast<float2*>(take(sizeof(float2) * (padded_rows / 32)));
             sb.max_chunks = sym_chunks(n_points);
             const size_t chunks = static_cast<size_t>(sb.max_chunks);
             sb.pool = static_cast<uint64_t*>(take(chunks * kChunkWords * sizeof(uint64_t)));
@@ -2989,7 +3115,7 @@ void mfma_knn(const float* keyrows, const float* centred, int stride, const floa
         column_thresholds_kernel<<<(padded_rows + 255) / 256, 256, 0, stream>>>(
             tseed, norms, madj, ord->order, heavy, n_points, padded_rows, c, sb.wcol);
         column_maxima_kernel<<<(2 * (padded_rows / 32) + 255) / 256, 256, 0, stream>>>(
-            sb.wcol, padded_rows / 32, sb.wmax);
+            sb.wcol, madj, padded_rows / 32, sb.wmax, sb.nmax);
         CULOF_CHECK(hipGetLastError());
         auto go_bf16 = [&](auto D) {
             if constexpr (decltype(D)::value < kWideMin) launch_sym<ScreenBF16, decltype(D)::value>(args, sb, stream);

@@ -323,6 +323,68 @@ def test_split_bf16_screen_bound_holds(name):
     assert worst_total <= 1.0, worst_total
 
 
+def _coarse_slack(d):
+    """The two-level BF16 screen's slack, as the kernels state it: the lo x hi
+    and hi x lo corrections are at most 2^-7 (1 + 2^-8) sum |a_t b_t|, their
+    2 d FP32 additions at most 2 d x 2u x 3.1 (N_q + N_c); x 1.01 + 16u."""
+    u = 2.0 ** -24
+    return 1.01 * (2.0 ** -7 * (1 + 2.0 ** -8) + 12.4 * d * u) + 16 * u
+
+
+@pytest.mark.parametrize("name", ["far", "clusters", "duplicates", "wide", "boundary"])
+def test_two_level_bf16_coarse_slack_holds(name):
+    """The coarse value of the two-level BF16 screen (m_c plus the hi x hi
+    products, FP32 accumulation) stays within g (N_q + N_c) of the full value
+    (the same accumulator continued with the lo x hi and hi x lo products), so
+    a tile whose coarse values all exceed a threshold widened by that margin
+    holds no pair the full screen passes. Values just off the bf16 rounding
+    midpoints make the residuals as large as they get (|lo(x)| near 2^-8
+    |x|); the same check with the residual bound taken as 2^-9 must fail."""
+    sets = dict(_adversarial_sets())
+    rng = np.random.default_rng(9)
+    mant = (1.0 + (rng.integers(0, 128, size=(64, 48)) + 0.5 + rng.uniform(-1e-3, 1e-3, (64, 48))) / 128)
+    sets["boundary"] = (mant * 2.0 ** rng.integers(-10, 10, size=(64, 48)) *
+                        rng.choice([-1.0, 1.0], size=(64, 48))).astype(np.float32)
+    X = sets[name]
+    if name != "boundary":
+        xc, norms, _, _ = oracle.center(X)
+    else:
+        xc, norms = X, (X.astype(np.float64) ** 2).sum(1)
+    xc = np.asarray(xc, np.float32)
+    nn = np.asarray(norms, np.float64)
+    d = xc.shape[1]
+    hi = _bf16(xc)
+    lo = _bf16((xc - hi).astype(np.float32))
+    c = 3.1 * 2.0 ** -16 + (12 * d + 64) * 2.0 ** -24
+    madj = (nn * (1 - c)).astype(np.float32)
+    worst = worst_tight = 0.0
+    for i in range(0, xc.shape[0], max(1, xc.shape[0] // 24)):
+        b = (-2.0 * xc[i]).astype(np.float32)
+        bh = _bf16(b)
+        bl = _bf16((b - bh).astype(np.float32))
+        p_hh = hi.astype(np.float64) * bh                       # exact in FP64 (and in FP32)
+        p_cor = np.concatenate([lo.astype(np.float64) * bh, hi.astype(np.float64) * bl], axis=1)
+        scale = nn + nn[i]
+        sel = scale > 0
+        for order in range(2):
+            acc = madj.copy()
+            cols = np.arange(d) if order == 0 else rng.permutation(d)
+            for t in cols:
+                acc = (acc + p_hh[:, t].astype(np.float32)).astype(np.float32)
+            coarse = acc.astype(np.float64)
+            cols = np.arange(2 * d) if order == 0 else rng.permutation(2 * d)
+            for t in cols:
+                acc = (acc + p_cor[:, t].astype(np.float32)).astype(np.float32)
+            gap = np.abs(acc.astype(np.float64) - coarse)
+            worst = max(worst, float((gap[sel] / (_coarse_slack(d) * scale[sel])).max(initial=0)))
+            # a residual bound of 2^-9 (half the true one) is too small
+            tight = 2.0 ** -8 * (1 + 2.0 ** -8) + 12.4 * d * 2.0 ** -24
+            worst_tight = max(worst_tight, float((gap[sel] / (tight * scale[sel])).max(initial=0)))
+    assert worst <= 1.0, worst
+    if name == "boundary":
+        assert worst_tight > 1.0, worst_tight
+
+
 @pytest.mark.parametrize("normalize", [False, True])
 def test_novelty_oracle_matches_sklearn(normalize):
     """oracle_lof_novelty against scikit-learn's LocalOutlierFactor(novelty=True)
