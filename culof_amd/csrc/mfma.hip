@@ -1258,7 +1258,9 @@ mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2
     // min(coarse) - Wmax <= L_q + g (N_q + N_c) plus the rounding of the
     // column difference (column test); N_c <= nmax / (1 - c) and the 1.01 in
     // g covers that factor. gq holds the per-query part of the margin (with
-    // 8u of |Tn| and |L_q| for the roundings of the widened thresholds).
+    // 8u of |Tn| and |L_q| for the roundings of the widened thresholds; a
+    // row that tests nothing, Tn = -inf, adds no |Tn|, which would make its
+    // column margin infinite).
     float gq[QT];
     constexpr float g = QT == 2 && DP >= 128 ? ScreenBF16::coarse_slack(DP) : 0.0f;
     const float c = p.c;
@@ -1273,7 +1275,7 @@ mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2
         Tn[t] = tests ? (__builtin_fmaf(tq, 1.0f + c, c * nq) - nq) + 4.0f * kU * (tq + nq)
                       : -__uint_as_float(kInfBits);
         Lq[t] = valid[t] ? 4.0f * kU * nq - p.madj[q[t]] : __uint_as_float(0x7FC00000u);
-        gq[t] = g * nq + 8.0f * kU * (__builtin_fabsf(Tn[t]) + __builtin_fabsf(Lq[t]));
+        gq[t] = g * nq + 8.0f * kU * ((tests ? __builtin_fabsf(Tn[t]) : 0.0f) + __builtin_fabsf(Lq[t]));
     }
     const int h_b = (nb - 1) / 2 + ((nb % 2 == 0 && B < nb / 2) ? 1 : 0);
     const long long span = h_b + 1;  // offsets 0..h_b

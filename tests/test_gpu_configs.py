@@ -12,6 +12,9 @@ structure fixed from the forward.
     plain-20k   clusters, n=20,000,  d=8,   k=20, contamination 0.01
     plain-200k  clusters, n=200,000, d=8,   k=20
     bf16-100k   low-rank, n=100,000, d=128, k=50, BF16 screen vs FP32
+                (and, for the two-level BF16 screen's margins, the same
+                generator with far rows and the reference's benchmark
+                distribution at d = 128)
     grad-50k    clusters, n=50,000,  d=32,  k=20, forward + backward
     ties-500k   quantised clusters with duplicates, n=500,000, d=64, k=32,
                 both forms x both tie rules, forward + backward
@@ -166,6 +169,37 @@ def test_bf16_100k_forms_identical_and_rows_exact():
     rows = np.random.default_rng(3).choice(X.shape[0], 200, replace=False)
     rows = np.concatenate([rows, np.argsort(-fp32["scores"])[:20]])
     _rows_match_oracle(X, fp32, rows, k, name)
+
+
+@pytest.mark.parametrize("data", ["lowrank-far", "reference-blobs"])
+def test_bf16_two_level_screen_far_rows(data):
+    """The two-level BF16 screen of the symmetric pass at 128 features where
+    row norms and seeds spread widely, so its coarse margins (which grow with
+    the norms and the column thresholds) are tested at large values: the
+    low-rank workload with 2 % of its rows replaced by points uniform in
+    [-30, 30]^128, and the reference's benchmark distribution (4 unit Gaussian
+    clusters, 1 % uniform outliers in [-20, 20]^d) at n = 100k, d = 128. Both
+    forms give identical results; the far rows and sampled others are
+    bit-exact with the oracle."""
+    from culof_amd.benchmark import blobs_with_outliers
+
+    rng = np.random.default_rng(31)
+    if data == "lowrank-far":
+        X = workloads.lowrank(n=100_000, d=128, seed=31)
+        far = rng.choice(X.shape[0], 2_000, replace=False)
+        X[far] = rng.uniform(-30.0, 30.0, size=(far.size, X.shape[1])).astype(np.float32)
+    else:
+        X = blobs_with_outliers(100_000, 128, seed=3)
+        far = np.arange(1_000)  # the outliers come first
+    k = 20
+    print(f"two-level {data}: n={X.shape[0]} d={X.shape[1]} k={k}", flush=True)
+    x = torch.from_numpy(X).cuda()
+    fp32 = _host(_state(x, k, "fp32"))
+    bf16 = _host(_state(x, k, "bf16"))
+    for key in fp32:
+        _same_bits(bf16[key], fp32[key], f"two-level {data} bf16 vs fp32 {key}")
+    rows = np.concatenate([far[:20], rng.choice(X.shape[0], 40, replace=False)])
+    _rows_match_oracle(X, bf16, rows, k, f"two-level {data}")
 
 
 # ----------------------------------------------------------------- grad-50k
