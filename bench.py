@@ -61,6 +61,7 @@ sys.path.insert(0, ROOT)
 METRIC = "LOF pts/s (n=200k,d=8,k=20) + fwd+bwd pts/s; bf16-screen vs fp32 at equal kNN"
 FP32_PEAK_TFLOPS = 157.3      # MI355X dense FP32 (matrix and packed vector)
 FP32_VALU_TFLOPS = 78.6       # non-packed FP32 VALU (packed FP32 ops are disabled at compile time)
+BF16_PEAK_TFLOPS = 2500.0     # MI355X dense BF16 matrix
 HBM_PEAK_GBS = 8000.0
 REFERENCE_PTS_S = 1.42e5      # reference on RTX A6000 at n=200k, d=8, k=20 (its README.md:50)
 KERNELS = ("center_partials", "center_apply", "order", "seed", "order_split", "heavy_rows", "screen_prep",
@@ -82,6 +83,8 @@ def parse_args():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="plain-200k")
+    p.add_argument("--form", choices=("fp32", "bf16"), default="fp32",
+                   help="screen form of the timed forward (the headline value is the plain FP32 form)")
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--cpu-rows", type=int, default=2000,
                    help="query rows in the CPU-oracle sample (0 skips the leg)")
@@ -328,9 +331,11 @@ def main() -> int:
     lib = _native.lib()
     stream = _native.current_stream()
 
+    form_code = _native.FORM_PLAIN if args.form == "fp32" else _native.FORM_BF16
+
     def step():
         _native.check(lib.culof_lof_device(
-            x.data_ptr(), n, d, k, 0, _native.FORM_PLAIN, scores.data_ptr(), idx.data_ptr(),
+            x.data_ptr(), n, d, k, 0, form_code, scores.data_ptr(), idx.data_ptr(),
             d2.data_ptr(), kd.data_ptr(), lrd.data_ptr(), stream))
 
     for _ in range(args.warmup):
@@ -366,12 +371,28 @@ def main() -> int:
     executed = (tile_pairs / max(sweep_launches, 1) * 32.0 * 32.0 * 2.0 * dp) if tile_pairs else all_pairs_flops
     achieved = executed / avg_sweep_s / 1e12
     effective = 2.0 * d * float(n) * float(n) / avg_sweep_s / 1e12
+    if args.form == "bf16":
+        # split BF16 screen: 3 bf16 MFMA products per pair and feature, rows
+        # padded to at least 16 features; against the dense BF16 matrix peak
+        # (at 128 features the two-level screen runs the 2 correction
+        # products only for the tile pairs its coarse test keeps, so this
+        # counts more than it executes there)
+        dpb = max(16, 1 << max(2, (d - 1).bit_length())) if d <= 128 else dp
+        executed = executed / dp * dpb * 3.0
+        all_pairs_flops = all_pairs_flops / dp * dpb * 3.0
+        achieved = executed / avg_sweep_s / 1e12
+    peak = FP32_PEAK_TFLOPS if args.form == "fp32" else BF16_PEAK_TFLOPS
+    form_note = ("FP32 matrix-core flops the symmetric collect pass executes per launch (2*dp per row "
+                 "pair" if args.form == "fp32" else
+                 "BF16 matrix-core flops of the split screen per launch (at 128 features the two-level "
+                 "screen runs 2 of the 3 products only where its coarse test passes, so this overcounts "
+                 "there; 3 x 2*dp per row pair, rows padded to at least 16 features,")
     roofline = {
         "bound": "mfma",
         "achieved": round(achieved, 3),
-        "peak": FP32_PEAK_TFLOPS,
+        "peak": peak,
         "unit": "TFLOP/s",
-        "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+        "frac": round(achieved / peak, 4),
         "traffic": read_traffic(dominant, args.workload),
         "kernel": dominant,
         "avg_launch_ms": round(avg_sweep_s * 1e3, 4),
@@ -379,12 +400,11 @@ def main() -> int:
         "tile_pairs_per_launch": round(tile_pairs / max(sweep_launches, 1)) if tile_pairs else None,
         "kept_fraction_of_unordered_pairs": round(executed / all_pairs_flops, 4),
         "effective_tflops": round(effective, 3),
-        "effective_frac": round(effective / FP32_PEAK_TFLOPS, 4),
-        "note": "achieved = FP32 matrix-core flops the symmetric collect pass executes per "
-                "launch (2*dp per row pair of every 32x32 tile pair it screens: each unordered "
+        "effective_frac": round(effective / peak, 4),
+        "note": "achieved = " + form_note + " of every 32x32 tile pair it screens: each unordered "
                 "pair at most once, and only the tile pairs its box pruning keeps, counted on "
                 "the device; flops_per_launch) / its average HIP-event launch time, against the "
-                "FP32 matrix peak; effective_* count 2*d*n^2 (every ordered pair, the reference's "
+                "form's matrix peak; effective_* count 2*d*n^2 (every ordered pair, the reference's "
                 "Gram contraction) over the same time; traffic = HBM bytes per launch from the "
                 "committed FETCH_SIZE/WRITE_SIZE summary",
     }
@@ -491,7 +511,7 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": args.workload, "n": n, "d": d, "k": k, "seed": args.seed,
-                   "generator": spec["gen"], "device_resident_input": True},
+                   "generator": spec["gen"], "form": args.form, "device_resident_input": True},
         "roofline": roofline,
         "hbm": hbm,
         "cpu_baseline": cpu_baseline,
