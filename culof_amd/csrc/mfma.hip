@@ -1392,7 +1392,10 @@ mfma_sym_collect_kernel(MfmaArgs p, const float* __restrict__ wcol, const float2
 // tiles whose box lies farther from the block's query box than both the
 // block's and the chunk's largest seeds (the lower bound, rounded, scaled
 // down by (2 DP + 16) 2^-23 to cover its own rounding and the exact keys')
-// holds no such pair and is skipped. Results do not change.
+// holds no such pair and is skipped. Results do not change. Boxes are
+// stored feature-major: the lower corners [DP][tiles], then the upper
+// corners [DP][tiles] (groups likewise), so the box tests, one tile per
+// thread, read consecutive words.
 template <int DP>
 __global__ void __launch_bounds__(256)
 tile_bounds_kernel(const float* __restrict__ xs_ord, const int* __restrict__ order,
@@ -1413,8 +1416,8 @@ tile_bounds_kernel(const float* __restrict__ xs_ord, const int* __restrict__ ord
         hi = __builtin_fmaxf(hi, v);
         if (f == 0 && !heavy[pos]) tm = __builtin_fmaxf(tm, tseed[order[pos]]);
     }
-    box[(static_cast<size_t>(t) * 2) * DP + f] = lo;
-    box[(static_cast<size_t>(t) * 2 + 1) * DP + f] = hi;
+    box[static_cast<size_t>(f) * tiles + t] = lo;
+    box[static_cast<size_t>(DP + f) * tiles + t] = hi;
     if (f == 0) tmax[t] = tm;
 }
 
@@ -1435,34 +1438,26 @@ group_bounds_kernel(const float* __restrict__ box, const float* __restrict__ tma
     float hi = -__uint_as_float(kInfBits);
     float tm = -__uint_as_float(kInfBits);
     for (int t = g * kPruneGroup; t < min(tiles, (g + 1) * kPruneGroup); ++t) {
-        lo = __builtin_fminf(lo, box[(static_cast<size_t>(t) * 2) * DP + f]);
-        hi = __builtin_fmaxf(hi, box[(static_cast<size_t>(t) * 2 + 1) * DP + f]);
+        lo = __builtin_fminf(lo, box[static_cast<size_t>(f) * tiles + t]);
+        hi = __builtin_fmaxf(hi, box[static_cast<size_t>(DP + f) * tiles + t]);
         if (f == 0) tm = __builtin_fmaxf(tm, tmax[t]);
     }
-    gbox[(static_cast<size_t>(g) * 2) * DP + f] = lo;
-    gbox[(static_cast<size_t>(g) * 2 + 1) * DP + f] = hi;
+    gbox[static_cast<size_t>(f) * groups + g] = lo;
+    gbox[static_cast<size_t>(DP + f) * groups + g] = hi;
     if (f == 0) gtmax[g] = tm;
 }
 
-// Squared gap between two boxes (lo / hi rows of DP floats; the second in LDS).
+// Squared gap between two boxes: the first feature-major in global memory
+// (feature f of its lower corner at lo[f * stride], of its upper corner at
+// hi[f * stride]; the threads of a wave read consecutive boxes, so each load
+// is coalesced), the second in LDS (DP floats each).
 template <int DP>
 __device__ __forceinline__ float box_gap2(const float* __restrict__ lo, const float* __restrict__ hi,
-                                          const float* blo, const float* bhi) {
-    const float4* lo4 = reinterpret_cast<const float4*>(lo);
-    const float4* hi4 = reinterpret_cast<const float4*>(hi);
-    const float4* blo4 = reinterpret_cast<const float4*>(blo);
-    const float4* bhi4 = reinterpret_cast<const float4*>(bhi);
+                                          size_t stride, const float* blo, const float* bhi) {
     float lb = 0.0f;
 #pragma unroll 8
-    for (int f4 = 0; f4 < DP / 4; ++f4) {
-        const float4 l = lo4[f4], hh = hi4[f4], bl = blo4[f4], bh = bhi4[f4];
-        float g = __builtin_fmaxf(0.0f, __builtin_fmaxf(l.x - bh.x, bl.x - hh.x));
-        lb = __builtin_fmaf(g, g, lb);
-        g = __builtin_fmaxf(0.0f, __builtin_fmaxf(l.y - bh.y, bl.y - hh.y));
-        lb = __builtin_fmaf(g, g, lb);
-        g = __builtin_fmaxf(0.0f, __builtin_fmaxf(l.z - bh.z, bl.z - hh.z));
-        lb = __builtin_fmaf(g, g, lb);
-        g = __builtin_fmaxf(0.0f, __builtin_fmaxf(l.w - bh.w, bl.w - hh.w));
+    for (int f = 0; f < DP; ++f) {
+        const float g = __builtin_fmaxf(0.0f, __builtin_fmaxf(lo[f * stride] - bhi[f], blo[f] - hi[f * stride]));
         lb = __builtin_fmaf(g, g, lb);
     }
     return lb;
@@ -1494,7 +1489,8 @@ sym_live_chunks_kernel(const float* __restrict__ box, const float* __restrict__ 
     const long long span = h_b + 1;
     const int o0 = static_cast<int>(span * y / ns);
     const int o1 = static_cast<int>(span * (y + 1) / ns);
-    const int wrap = kBlockTiles * nb;
+    const int wrap = kBlockTiles * nb;  // tiles (the box arrays' stride)
+    const int n_groups = (wrap + kPruneGroup - 1) / kPruneGroup;
     const int t_begin = kBlockTiles * (B + o0);
     const int t_end = kBlockTiles * (B + o1);
     const int nc = (t_end - t_begin + CT - 1) / CT;
@@ -1502,8 +1498,8 @@ sym_live_chunks_kernel(const float* __restrict__ box, const float* __restrict__ 
         float lo = __uint_as_float(kInfBits);
         float hi = -__uint_as_float(kInfBits);
         for (int t = kBlockTiles * B; t < kBlockTiles * (B + 1); ++t) {
-            lo = __builtin_fminf(lo, box[(static_cast<size_t>(t) * 2) * DP + f]);
-            hi = __builtin_fmaxf(hi, box[(static_cast<size_t>(t) * 2 + 1) * DP + f]);
+            lo = __builtin_fminf(lo, box[static_cast<size_t>(f) * wrap + t]);
+            hi = __builtin_fmaxf(hi, box[static_cast<size_t>(DP + f) * wrap + t]);
         }
         s_lo[f] = lo;
         s_hi[f] = hi;
@@ -1530,9 +1526,10 @@ sym_live_chunks_kernel(const float* __restrict__ box, const float* __restrict__ 
             // bound is at least the tile's): most far tiles fall away there,
             // and the group's bounds are the same loads for its 8 threads
             const size_t pg = pt / kPruneGroup;
-            const float lbg = box_gap2<DP>(gbox + (pg * 2) * DP, gbox + (pg * 2 + 1) * DP, s_lo, s_hi);
+            const float lbg = box_gap2<DP>(gbox + pg, gbox + static_cast<size_t>(DP) * n_groups + pg, n_groups,
+                                           s_lo, s_hi);
             if (!(lbg * shrink > __builtin_fmaxf(s_tq, gtmax[pg]))) {
-                const float lb = box_gap2<DP>(box + (pt * 2) * DP, box + (pt * 2 + 1) * DP, s_lo, s_hi);
+                const float lb = box_gap2<DP>(box + pt, box + static_cast<size_t>(DP) * wrap + pt, wrap, s_lo, s_hi);
                 const float thr = __builtin_fmaxf(s_tq, tmax[pt]);
                 ok = !(lb * shrink > thr);  // NaN keeps
             }
