@@ -34,6 +34,14 @@ constexpr int kPiv = 64;     // pivots per level
 constexpr int kBlock = 256;
 constexpr int kKeys = kPiv * kPiv;
 
+// Orders a wave's LDS writes before its later reads (and reads before later
+// writes) without a block barrier.
+__device__ __forceinline__ void seed_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Index of the nearest of `count` pivot rows (LDS or global) to row x, by
 // difference-form FP32 squared distance; ties to the lower pivot.
 template <int DP>
@@ -248,6 +256,7 @@ seed_kernel(const float* __restrict__ xs_ord, const int* __restrict__ order, int
     constexpr bool kScalarQ = DP >= 32;
     __shared__ __attribute__((aligned(16))) float s_q[kScalarQ ? 1 : kQ][kScalarQ ? 1 : DP];
     __shared__ __attribute__((aligned(16))) float s_c[kRows * kPitch];
+    __shared__ uint32_t s_few[4][64];  // a wave's keys <= M, compacted
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int p0 = blockIdx.x * kQ;
@@ -359,17 +368,64 @@ seed_kernel(const float* __restrict__ xs_ord, const int* __restrict__ order, int
     // above, so the search stops kSeedLowBits short: the prefix P found has
     // P <= K < P + 2^kSeedLowBits, and P with its low bits set is returned
     // (at most 2^-13 relative above K; a third fewer search steps).
+    //
+    // The search runs over the few keys that can matter: M, the k-th
+    // smallest of the 64 lane minima, bounds K (the k lanes whose minima are
+    // at most M hold k keys at most M), so K is the k-th smallest of the keys
+    // <= M. Every probe of the search gets the same answer from those keys
+    // as from all of them (a probe at most M + 1 counts the same keys below
+    // it; one above M finds at least k either way), so the result is the
+    // same bits. When at most 64 keys are <= M they are compacted one per
+    // lane through LDS and searched with one compare per step instead of
+    // W / 64; otherwise (many equal keys) the search runs over all keys.
     constexpr int kSeedLowBits = 10;
 #pragma unroll
     for (int t = 0; t < QPW; ++t) {
         const int pos = p0 + wave * QPW + t;
         uint32_t ans = 0;
-        for (int b = 30; b >= kSeedLowBits; --b) {
-            const uint32_t probe = ans | (1u << b);
-            int below = 0;
+        bool searched = false;
+        if (k <= 64) {
+            uint32_t m = key[t][0];
 #pragma unroll
-            for (int u = 0; u < W / 64; ++u) below += __popcll(__ballot(key[t][u] < probe));
-            if (below < k) ans = probe;
+            for (int u = 1; u < W / 64; ++u) m = min(m, key[t][u]);
+            uint32_t M = 0;  // the k-th smallest lane minimum
+            for (int b = 30; b >= 0; --b) {
+                const uint32_t probe = M | (1u << b);
+                if (__popcll(__ballot(m < probe)) < k) M = probe;
+            }
+            int c = 0;
+#pragma unroll
+            for (int u = 0; u < W / 64; ++u) c += key[t][u] <= M ? 1 : 0;
+            int incl = c;  // inclusive prefix over the lanes
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int y = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += y;
+            }
+            const int total = __shfl(incl, 63, 64);
+            if (total <= 64) {
+                int at = incl - c;
+#pragma unroll
+                for (int u = 0; u < W / 64; ++u)
+                    if (key[t][u] <= M) s_few[wave][at++] = key[t][u];
+                seed_wave_sync();
+                const uint32_t v = lane < total ? s_few[wave][lane] : 0xFFFFFFFFu;
+                seed_wave_sync();  // the next query's writes come after every lane's read
+                for (int b = 30; b >= kSeedLowBits; --b) {
+                    const uint32_t probe = ans | (1u << b);
+                    if (__popcll(__ballot(v < probe)) < k) ans = probe;
+                }
+                searched = true;
+            }
+        }
+        if (!searched) {
+            for (int b = 30; b >= kSeedLowBits; --b) {
+                const uint32_t probe = ans | (1u << b);
+                int below = 0;
+#pragma unroll
+                for (int u = 0; u < W / 64; ++u) below += __popcll(__ballot(key[t][u] < probe));
+                if (below < k) ans = probe;
+            }
         }
         ans |= (1u << kSeedLowBits) - 1u;
         if (lane == 0 && pos < n) tseed[order[pos]] = __uint_as_float(ans > kInfBits ? kInfBits : ans);
