@@ -393,7 +393,8 @@ def main() -> int:
         "peak": peak,
         "unit": "TFLOP/s",
         "frac": round(achieved / peak, 4),
-        "traffic": read_traffic(dominant, args.workload),
+        # (the committed counter summaries are of the plain FP32 form)
+        "traffic": read_traffic(dominant, args.workload) if args.form == "fp32" else None,
         "kernel": dominant,
         "avg_launch_ms": round(avg_sweep_s * 1e3, 4),
         "flops_per_launch": executed,
@@ -409,7 +410,7 @@ def main() -> int:
                 "committed FETCH_SIZE/WRITE_SIZE summary",
     }
     hbm = {}
-    for scope, kname in HBM_KERNELS.items():
+    for scope, kname in (HBM_KERNELS.items() if args.form == "fp32" else ()):
         ms, launches = total_kernel.get(scope, (0.0, 0))
         nbytes = kernel_bytes(dominant, kname, args.workload)
         if not launches or nbytes is None:

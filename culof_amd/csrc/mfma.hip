@@ -448,12 +448,13 @@ __host__ __device__ constexpr int stage_words(int k) { return (2 * k + 2 + 3) & 
 // to whole tiles (zero rows, NaN norms and thresholds); tiles past t_end are
 // never visited. `stage` holds 2 * ChunkShape::kBytes (kSymBytes with kSym).
 //
-// Two query tiles per wave (QT = 2, the BF16 screen) screen in two levels:
-// coarse(acc, v, wmax, nmax, t) gets each query tile's accumulator after the
-// hi x hi products alone, with this half's maximum column threshold and
-// maximum adjusted norm over the tile (nmax, staged like wmax), and returns
-// whether the tile can hold a pair that passes; only then are the correction
-// products added and the tile visited.
+// With two query tiles per wave (QT = 2, the BF16 screen) at 128 features
+// or more the screen runs in two levels: coarse(acc, v, wmax, nmax, t) gets
+// each query tile's accumulator after the hi x hi products alone, with this
+// half's maximum column threshold and maximum adjusted norm over the tile
+// (nmax, staged like wmax), and returns whether the tile can hold a pair
+// that passes; only then are the correction products added and the tile
+// visited. Other shapes never call coarse.
 template <class Op, int DP, bool kSym, int QT = 1, class Visit, class Coarse>
 __device__ __forceinline__ void stream_range(const MfmaArgs& p, const typename Op::template Frag<DP>* bq,
                                              int j, int h, unsigned char* stage, int t_begin, int t_end,
