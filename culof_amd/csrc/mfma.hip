@@ -2059,6 +2059,32 @@ __device__ __forceinline__ void wide_select(const MfmaArgs& p, uint64_t* s_words
     }
 }
 
+// The list pass's partial lists, a wave per row: the row's slices x k words
+// (each slice's list ascending, padded with sentinels; every candidate in one
+// slice) read coalesced into LDS, and the k smallest emitted in the
+// reference's slot order by the wave select's own search (the same words and
+// the same order as a serial merge of the lists). slices x k <= 64 WPL.
+template <int WPL>
+__global__ void __launch_bounds__(256)
+merge_slices_wave_kernel(MfmaArgs p) {
+    __shared__ uint64_t s_words[4][64 * WPL];
+    __shared__ uint64_t s_sel[4][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int local = blockIdx.x * 4 + wave;
+    const int rows = p.row_count ? min(p.rows, *p.row_count) : p.rows;
+    if (local >= rows) return;  // whole waves; no block barrier below
+    const int qi = p.qlist ? p.qlist[local] : p.row0 + local;
+    const int k = p.k;
+    const int total = p.slices * k;
+    for (int e = lane; e < total; e += 64) {
+        const int s = e / k;
+        s_words[wave][e] = p.partial[(static_cast<size_t>(s) * p.rows + local) * k + (e - s * k)];
+    }
+    wave_lds_sync();
+    select_words<WPL>(p, s_words[wave], total, qi, lane, s_sel[wave]);
+}
+
 __device__ __forceinline__ int query_of(const MfmaArgs& p, int local) {
     return p.qlist ? p.qlist[local] : p.row0 + local;
 }
@@ -2553,9 +2579,23 @@ void launch_mfma(const MfmaArgs& a, hipStream_t stream) {
     }
     if (a.slices > 1) {
         ProfileScope prof("merge_slices", stream);
-        merge_slices_kernel<KMAX><<<(a.rows + 63) / 64, 64, 0, stream>>>(
-            a.partial, a.slices, a.rows, a.row0, a.qlist, a.row_count, a.n, a.k, a.nbr_idx, a.nbr_d2, a.kdist2,
-            a.fb_rows, a.fb_count, a.fb_cap);
+        const int words = a.slices * a.k;
+        const unsigned wgrid = static_cast<unsigned>((a.rows + 3) / 4);
+        MfmaArgs m = a;
+        m.stage = nullptr;  // the merged lists go straight to the outputs
+        if (words <= 128) {
+            merge_slices_wave_kernel<2><<<wgrid, 256, 0, stream>>>(m);
+        } else if (words <= 256) {
+            merge_slices_wave_kernel<4><<<wgrid, 256, 0, stream>>>(m);
+        } else if (words <= 512) {
+            merge_slices_wave_kernel<8><<<wgrid, 256, 0, stream>>>(m);
+        } else if (words <= 1024) {
+            merge_slices_wave_kernel<16><<<wgrid, 256, 0, stream>>>(m);
+        } else {
+            merge_slices_kernel<KMAX><<<(a.rows + 63) / 64, 64, 0, stream>>>(
+                a.partial, a.slices, a.rows, a.row0, a.qlist, a.row_count, a.n, a.k, a.nbr_idx, a.nbr_d2,
+                a.kdist2, a.fb_rows, a.fb_count, a.fb_cap);
+        }
     }
 }
 
