@@ -1569,7 +1569,7 @@ __global__ void add_counter_kernel(unsigned long long* __restrict__ c, unsigned 
 // slice S + y of their rows: one atomic per word reserves its popcount, then
 // the queries t * 32 + bit are written. Rows whose words found the pool
 // exhausted are marked overflowed (their first column slice is set past cap).
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kChunkWords)
 scatter_pairs_kernel(const uint64_t* __restrict__ pool, const int* __restrict__ chunk_region,
                      const int* __restrict__ chunk_count, const int* __restrict__ next_chunk,
                      int max_chunks, int ns, int cap, const int* __restrict__ guard,
@@ -1582,7 +1582,7 @@ scatter_pairs_kernel(const uint64_t* __restrict__ pool, const int* __restrict__ 
         const uint64_t* src = pool + static_cast<size_t>(ch) * kChunkWords;
         const uint32_t q0 = static_cast<uint32_t>(r / ns) * 32;
         const int y = r % ns;
-        for (int e = threadIdx.x; e < count; e += 256) {
+        for (int e = threadIdx.x; e < count; e += kChunkWords) {  // one thread per word
             const uint64_t w = src[e];
             uint32_t m = static_cast<uint32_t>(w);
             const size_t slot = static_cast<size_t>(w >> 32) * (2 * ns) + ns + y;
@@ -2702,7 +2702,7 @@ void launch_sym(const MfmaArgs& a, const SymBuffers& sb, hipStream_t stream) {
     }
     {
         ProfileScope prof("scatter_pairs", stream);
-        scatter_pairs_kernel<<<4096, 256, 0, stream>>>(sb.pool, sb.chunk_region, sb.chunk_count,
+        scatter_pairs_kernel<<<8192, kChunkWords, 0, stream>>>(sb.pool, sb.chunk_region, sb.chunk_count,
                                                       sb.next_chunk, sb.max_chunks, sb.slices, a.cap,
                                                       a.guard, a.cbuf, a.ccount);
         mark_overflow_kernel<<<(a.n + 255) / 256, 256, 0, stream>>>(sb.col_over, a.n, sb.slices, a.cap, a.ccount);
