@@ -294,6 +294,22 @@ def test_pruned_symmetric_pass_gives_same_bits(monkeypatch, n, d, k, form):
         _assert_same_bits(pruned[2][r], ok[0], f"row {r} kdist2")
 
 
+def test_two_level_screen_on_normalized_rows():
+    """z-scored rows (normalize) of 128 features through the symmetric pass:
+    the BF16 form's two-level screen gives the FP32 form's bits, on low-rank
+    clusters scaled and shifted far from unit variance (so the z-scoring
+    changes the norms the margins scale with)."""
+    from _gpu import knn
+    from culof_amd import workloads
+
+    X = (workloads.lowrank(n=34_000, d=128, seed=4) * np.float32(30.0) + np.float32(7.0)).astype(np.float32)
+    fp32 = knn(X, 20, normalize=True, form=0)
+    bf16 = knn(X, 20, normalize=True, form=1)
+    for a, b, name in zip(bf16, fp32, ("idx", "d2", "kdist2")):
+        np.testing.assert_array_equal(_bits(a) if a.dtype == np.float32 else a,
+                                      _bits(b) if b.dtype == np.float32 else b, err_msg=name)
+
+
 @pytest.mark.parametrize("form", [0, 1], ids=["fp32", "bf16"])
 def test_overflow_guard_in_symmetric_pass(form):
     """Norms beyond FLT_MAX / 16 set the device guard: the symmetric pass and
