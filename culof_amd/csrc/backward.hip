@@ -29,6 +29,7 @@
 #include "internal.hpp"
 
 #include <stdexcept>
+#include <type_traits>
 
 namespace culof {
 namespace detail {
@@ -167,46 +168,61 @@ __device__ void heap_sort(int* a, int len) {
     }
 }
 
-// The kBlock consecutive points of a block own one contiguous CSR range; when
-// it fits (kSortLds entries) it is sorted in LDS, each thread its own segment
-// (insertion sort for short ones), else each segment in place in global memory.
+// The kBlock consecutive points of a block own one contiguous CSR range. When
+// it fits (kSortLds entries) it is copied to LDS and every entry finds its
+// place in its own segment by counting the smaller entries there (the entries
+// of a segment are distinct): the work is spread over the block's lanes by
+// entry, so a hub point that sits in hundreds of lists costs L^2 / kSortThreads
+// steps per lane instead of a serial sort of L entries by one lane. Ranges that do
+// not fit sort each segment in place in global memory.
+// kSortThreads lanes share the ranking of the kBlock points' entries.
 constexpr int kSortLds = 8192;
+constexpr int kSortThreads = 1024;
 
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kSortThreads)
 rev_sort_kernel(int* __restrict__ rev, const int* __restrict__ offset,
                 const int* __restrict__ count, int n) {
     __shared__ int s_seg[kSortLds];
+    __shared__ int s_off[kBlock + 1];  // segment starts relative to the range, then its length
     const int p0 = blockIdx.x * kBlock;
-    const int p_last = min(p0 + kBlock, n) - 1;
+    const int pts = min(kBlock, n - p0);
     const int begin = offset[p0];
-    const int span = offset[p_last] + count[p_last] - begin;
+    const int span = offset[p0 + pts - 1] + count[p0 + pts - 1] - begin;
     const int p = p0 + threadIdx.x;
     if (span > kSortLds) {
-        if (p < n) heap_sort(rev + offset[p], count[p]);
+        if (static_cast<int>(threadIdx.x) < pts) heap_sort(rev + offset[p], count[p]);
         return;
     }
-    for (int e = threadIdx.x; e < span; e += kBlock) s_seg[e] = rev[begin + e];
+    if (static_cast<int>(threadIdx.x) < pts) s_off[threadIdx.x] = offset[p] - begin;
+    if (threadIdx.x == 0) s_off[pts] = span;
+    for (int e = threadIdx.x; e < span; e += kSortThreads) s_seg[e] = rev[begin + e];
     __syncthreads();
-    if (p < n) {
-        int* a = s_seg + (offset[p] - begin);
-        const int len = count[p];
-        if (len <= 32) {
-            for (int i = 1; i < len; ++i) {
-                const int v = a[i];
-                int j = i - 1;
-                while (j >= 0 && a[j] > v) {
-                    a[j + 1] = a[j];
-                    --j;
-                }
-                a[j + 1] = v;
-            }
-        } else {
-            heap_sort(a, len);
+    for (int q = threadIdx.x; q < span; q += kSortThreads) {
+        // the segment holding q: the last start <= q (empty segments share a
+        // start with the next one, whose start is then also <= q)
+        int lo = 0, hi = pts;  // s_off[lo] <= q < s_off[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_off[mid] <= q) lo = mid;
+            else hi = mid;
         }
+        const int seg_lo = s_off[lo];
+        const int seg_hi = s_off[lo + 1];
+        const int v = s_seg[q];
+        int rank = 0;
+#pragma unroll 4
+        for (int r = seg_lo; r < seg_hi; ++r) rank += s_seg[r] < v ? 1 : 0;
+        rev[begin + seg_lo + rank] = v;
     }
-    __syncthreads();
-    for (int e = threadIdx.x; e < span; e += kBlock) rev[begin + e] = s_seg[e];
 }
+
+// Reverse lists are walked kBatch entries at a time: the loads of a batch are
+// all issued before the first of its terms is added, so a long list costs one
+// memory latency per batch rather than per entry, while the terms are still
+// added one by one in list order. Past the end of a list a batch reads the
+// list's last entry again (the loop only runs for non-empty lists) and drops
+// its term.
+constexpr int kBatch = 8;
 
 // g_m_p = w_p S_p - lrd_p^2 sum_{i : p in N(i)} a_i
 __global__ void __launch_bounds__(kBlock)
@@ -218,7 +234,18 @@ grad_m_kernel(const int* __restrict__ rev, const int* __restrict__ offset,
     if (p >= n) return;
     double g_lrd = 0.0;
     const int* seg = rev + offset[p];
-    for (int r = 0; r < count[p]; ++r) g_lrd += a[seg[r] % n];
+    const int len = count[p];
+    for (int r0 = 0; r0 < len; r0 += kBatch) {
+        int e[kBatch];
+        double v[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) e[u] = seg[min(r0 + u, len - 1)];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) v[u] = a[e[u] % n];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+            if (r0 + u < len) g_lrd += v[u];
+    }
     gm[p] = static_cast<double>(w[p]) * S[p] - g_lrd * lrd[p] * lrd[p];
 }
 
@@ -231,18 +258,76 @@ grad_kd_kernel(const int* __restrict__ rev, const int* __restrict__ offset,
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     double s = 0.0;
+    const float kd2p = kd2[p];
     const int* seg = rev + offset[p];
-    for (int r = 0; r < count[p]; ++r) {
-        const int e = seg[r];
-        if (kd2[p] > d2[e]) s += gm[e % n] / k;
+    const int len = count[p];
+    for (int r0 = 0; r0 < len; r0 += kBatch) {
+        int e[kBatch];
+        float de[kBatch];
+        double g[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) e[u] = seg[min(r0 + u, len - 1)];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            de[u] = d2[e[u]];
+            g[u] = gm[e[u] % n];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+            if (r0 + u < len && kd2p > de[u]) s += g[u] / k;
     }
     gkd[p] = s;
 }
 
+// One 64-bit value moved between lanes by a DPP control (two 32-bit moves).
+template <int Ctrl>
+__device__ double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), Ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), Ctrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the G lanes of an aligned group, the same bits in every lane. A
+// butterfly: at each level every lane adds the total of the other half of its
+// block (after the level below, both halves hold their own totals in every
+// lane), so each lane forms A + B or B + A, the same value. Within a row of 16
+// lanes the partner comes through DPP (quad permutes, then the half-row and
+// row mirrors), above it through a lane permute.
+template <int G>
+__device__ double group_sum(double x) {
+    if constexpr (G >= 2) x += dpp_f64<0xB1>(x);   // quad_perm [1,0,3,2]
+    if constexpr (G >= 4) x += dpp_f64<0x4E>(x);   // quad_perm [2,3,0,1]
+    if constexpr (G >= 8) x += dpp_f64<0x141>(x);  // row_half_mirror
+    if constexpr (G >= 16) x += dpp_f64<0x140>(x); // row_mirror
+    if constexpr (G >= 32) x += __shfl_xor(x, 16, G);
+    if constexpr (G >= 64) x += __shfl_xor(x, 32, G);
+    return x;
+}
+
+// Each reverse entry e = slot * n + i (p = nbr[e] is the point whose list it
+// sits in) replaced by its query i, with the coefficient of its term in p's
+// gradient beside it: g_m_i / k when reach_ip took branch d, plus g_kd_i when
+// p is i's k-th neighbour.
+__global__ void __launch_bounds__(kBlock)
+rev_coef_kernel(int* __restrict__ rev, size_t nk, const int* __restrict__ nbr,
+                const float* __restrict__ d2, const float* __restrict__ kd2,
+                const double* __restrict__ gm, const double* __restrict__ gkd, int n, int k,
+                double* __restrict__ coef) {
+    const size_t r = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= nk) return;
+    const int e = rev[r];
+    const int i = e % n;
+    const int slot = e / n;
+    double c = 0.0;
+    if (!(kd2[nbr[e]] > d2[e])) c += gm[i] / k;  // branch d of reach_ip
+    if (slot == k - 1) c += gkd[i];             // p is the k-th neighbour of i
+    rev[r] = i;
+    coef[r] = c;
+}
+
 // A group of G lanes owns one point; lane l of the group holds features
 // l, l + G, ... (F of them), so rows are read coalesced and the FP64 row of
-// the gradient stays in registers. Squared distances are reduced across the
-// group with an xor butterfly, which leaves the same bits in every lane.
+// the gradient stays in registers.
 template <int G, int F>
 struct GroupRow {
     int lane;  // lane within the group
@@ -256,78 +341,179 @@ struct GroupRow {
         }
     }
 
-    // acc += c (x_a - x_b) / |x_a - x_b|; nothing when the points coincide.
-    __device__ void add_unit(const double (&xa)[F], const float* __restrict__ xb_row, double c,
-                             double (&acc)[F]) const {
-        if (c == 0.0) return;  // uniform across the group
-        double df[F];
-        double ss = 0.0;
+    // A row of another point as FP32 values, features past d as zeros.
+    __device__ void fetch(const float* __restrict__ row, float (&v)[F]) const {
 #pragma unroll
         for (int u = 0; u < F; ++u) {
             const int f = lane + G * u;
-            df[u] = xa[u] - ((f < d) ? static_cast<double>(xb_row[f]) : 0.0);
-            ss = fma(df[u], df[u], ss);
+            v[u] = (f < d) ? row[f] : 0.0f;
+        }
+    }
+
+    // acc += c_t (x_a - x_t) / |x_a - x_t| for t = 0 .. N-1, in that order; a
+    // term with c_t = 0, or whose points coincide, leaves acc as it is. No
+    // branches, so the N group sums overlap.
+    template <int N>
+    __device__ void add_terms(const double (&xa)[F], const float (&xb)[N][F], const double (&c)[N],
+                              double (&acc)[F]) const {
+        double df[N][F];
+        double ss[N];
+#pragma unroll
+        for (int t = 0; t < N; ++t) {
+            ss[t] = 0.0;
+#pragma unroll
+            for (int u = 0; u < F; ++u) {
+                df[t][u] = xa[u] - static_cast<double>(xb[t][u]);
+                ss[t] = fma(df[t][u], df[t][u], ss[t]);
+            }
         }
 #pragma unroll
-        for (int m = 1; m < G; m <<= 1) ss += __shfl_xor(ss, m, G);
-        if (ss == 0.0) return;
-        const double sc = c / sqrt(ss);
+        for (int t = 0; t < N; ++t) ss[t] = group_sum<G>(ss[t]);
 #pragma unroll
-        for (int u = 0; u < F; ++u) acc[u] = fma(sc, df[u], acc[u]);
+        for (int t = 0; t < N; ++t) {
+            const bool live = c[t] != 0.0 && ss[t] != 0.0;
+            const double sc = c[t] / sqrt(ss[t]);
+#pragma unroll
+            for (int u = 0; u < F; ++u) acc[u] = live ? fma(sc, df[t][u], acc[u]) : acc[u];
+        }
     }
 };
 
 // Every term a point owns or receives (own list, own k-distance, reverse
-// list) accumulated into its FP64 gradient row; O(n k d) in total.
+// list) accumulated into its FP64 gradient row; O(n k d) in total. Terms are
+// fetched kBatch at a time and added in list order.
+//
+// A point in many lists would leave its group running long after the rest, so
+// reverse lists are cut into chunks of kChunk entries: group p (p < n) takes
+// point p's own terms and the first chunk of its list; every further chunk is
+// extra chunk x, taken by group n + x, which writes a partial row that
+// chunk_finish_kernel adds to the point's row in chunk order.
+constexpr int kChunk = 64;
+
+// extra[p] = chunks of p's reverse list after the first
+__global__ void __launch_bounds__(kBlock)
+chunk_count_kernel(const int* __restrict__ count, int n, int* __restrict__ extra) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) extra[p] = count[p] > 0 ? (count[p] - 1) / kChunk : 0;
+}
+
+// owner[chunk_start[p] + t] = p for t < extra[p]
+__global__ void __launch_bounds__(kBlock)
+chunk_owner_kernel(const int* __restrict__ extra, const int* __restrict__ chunk_start, int n,
+                   int* __restrict__ owner) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    for (int t = 0; t < extra[p]; ++t) owner[chunk_start[p] + t] = p;
+}
+
 template <int G, int F>
 __global__ void __launch_bounds__(kBlock)
 grad_x_kernel(const float* __restrict__ xs, int d, const int* __restrict__ nbr,
               const float* __restrict__ d2, const float* __restrict__ kd2,
-              const int* __restrict__ rev, const int* __restrict__ offset,
-              const int* __restrict__ count, const double* __restrict__ gm,
-              const double* __restrict__ gkd, int n, int k, double* __restrict__ gx) {
-    const int p = (blockIdx.x * kBlock + threadIdx.x) / G;
-    if (p >= n) return;  // whole groups leave together
+              const int* __restrict__ src, const double* __restrict__ coef,
+              const int* __restrict__ offset, const int* __restrict__ count,
+              const double* __restrict__ gm, const double* __restrict__ gkd,
+              const int* __restrict__ extra, const int* __restrict__ chunk_start,
+              const int* __restrict__ owner, int n, int k, double* __restrict__ gx,
+              double* __restrict__ part) {
+    const int g = (blockIdx.x * kBlock + threadIdx.x) / G;  // whole groups take one branch
+    int p = g;
+    int chunk = 0;
+    if (g >= n) {
+        const int x = g - n;
+        if (x >= chunk_start[n - 1] + extra[n - 1]) return;
+        p = owner[x];
+        chunk = x - chunk_start[p] + 1;
+    }
     const GroupRow<G, F> row{static_cast<int>(threadIdx.x % G), d};
     double xp[F], acc[F];
     row.load(xs + static_cast<size_t>(p) * d, xp);
 #pragma unroll
     for (int u = 0; u < F; ++u) acc[u] = 0.0;
-    // own list: branch-d terms, then the own k-distance
-    const double own = gm[p] / k;
-    for (int t = 0; t < k; ++t) {
-        const size_t o = static_cast<size_t>(t) * n + p;
-        const int j = nbr[o];
-        if (!(kd2[j] > d2[o])) row.add_unit(xp, xs + static_cast<size_t>(j) * d, own, acc);
+    float xb[kBatch][F];
+    double c[kBatch];
+    // Every load of a batch is unconditional (past the end of a list it reads
+    // the list's last entry again) and the terms are picked with selects, so
+    // no load of a batch waits on another's value through a branch.
+    if (chunk == 0) {
+        // own list: branch-d terms, then the own k-distance
+        const double own = gm[p] / k;
+        for (int t0 = 0; t0 < k; t0 += kBatch) {
+            int j[kBatch];
+            float dt[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const size_t o = static_cast<size_t>(min(t0 + u, k - 1)) * n + p;
+                j[u] = nbr[o];
+                dt[u] = d2[o];
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const float kj = kd2[j[u]];
+                c[u] = (t0 + u < k && !(kj > dt[u])) ? own : 0.0;
+                row.fetch(xs + static_cast<size_t>(j[u]) * d, xb[u]);
+            }
+            row.template add_terms<kBatch>(xp, xb, c, acc);
+        }
+        float xk[1][F];
+        const double ck[1] = {gkd[p]};
+        row.fetch(xs + static_cast<size_t>(nbr[static_cast<size_t>(k - 1) * n + p]) * d, xk[0]);
+        row.template add_terms<1>(xp, xk, ck, acc);
     }
-    row.add_unit(xp, xs + static_cast<size_t>(nbr[static_cast<size_t>(k - 1) * n + p]) * d, gkd[p],
-                 acc);
-    // reverse list: p as neighbour j of query i
-    const int* seg = rev + offset[p];
-    const int len = count[p];
-    for (int r = 0; r < len; ++r) {
-        const int e = seg[r];
-        const int i = e % n;
-        const int slot = e / n;
-        double c = 0.0;
-        if (!(kd2[p] > d2[e])) c += gm[i] / k;  // branch d of reach_ip
-        if (slot == k - 1) c += gkd[i];         // p is the k-th neighbour of i
-        row.add_unit(xp, xs + static_cast<size_t>(i) * d, c, acc);
+    // reverse list: p as neighbour j of query i, coefficients precomputed
+    const int* sq = src + offset[p];
+    const double* cq = coef + offset[p];
+    const int r_lo = chunk * kChunk;
+    const int r_hi = min(r_lo + kChunk, count[p]);
+    for (int r0 = r_lo; r0 < r_hi; r0 += kBatch) {
+        int i[kBatch];
+        double ci[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int r = min(r0 + u, r_hi - 1);
+            i[u] = sq[r];
+            ci[u] = cq[r];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            row.fetch(xs + static_cast<size_t>(i[u]) * d, xb[u]);
+            c[u] = r0 + u < r_hi ? ci[u] : 0.0;
+        }
+        row.template add_terms<kBatch>(xp, xb, c, acc);
     }
+    double* out = chunk == 0 ? gx + static_cast<size_t>(p) * d : part + static_cast<size_t>(g - n) * d;
 #pragma unroll
     for (int u = 0; u < F; ++u) {
         const int f = row.lane + G * u;
-        if (f < d) gx[static_cast<size_t>(p) * d + f] = acc[u];
+        if (f < d) out[f] = acc[u];
     }
+}
+
+// gx[p] += the partial rows of p's extra chunks, in chunk order
+__global__ void __launch_bounds__(kBlock)
+chunk_finish_kernel(const int* __restrict__ extra, const int* __restrict__ chunk_start,
+                    const double* __restrict__ part, int n, int d, double* __restrict__ gx) {
+    const size_t cell = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (cell >= static_cast<size_t>(n) * d) return;
+    const int p = static_cast<int>(cell / d);
+    const int f = static_cast<int>(cell % d);
+    const int ex = extra[p];
+    if (ex == 0) return;
+    double v = gx[cell];
+    for (int t = 0; t < ex; ++t) v += part[static_cast<size_t>(chunk_start[p] + t) * d + f];
+    gx[cell] = v;
 }
 
 template <int G, int F>
 void launch_grad_x(const float* xs, int d, const int* nbr, const float* d2, const float* kd2,
-                   const int* rev, const int* offset, const int* count, const double* gm,
-                   const double* gkd, int n, int k, double* gx, hipStream_t s) {
-    const size_t lanes = static_cast<size_t>(n) * G;
+                   const int* src, const double* coef, const int* offset, const int* count,
+                   const double* gm, const double* gkd, const int* extra, const int* chunk_start,
+                   const int* owner, int n, int k, double* gx, double* part, hipStream_t s) {
+    // n point groups plus at most n k / kChunk extra chunks
+    const size_t groups = static_cast<size_t>(n) + static_cast<size_t>(n) * k / kChunk;
+    const size_t lanes = groups * G;
     grad_x_kernel<G, F><<<static_cast<unsigned>((lanes + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-        xs, d, nbr, d2, kd2, rev, offset, count, gm, gkd, n, k, gx);
+        xs, d, nbr, d2, kd2, src, coef, offset, count, gm, gkd, extra, chunk_start, owner, n, k, gx, part);
 }
 
 // Per-feature sums for the z-score chain: sum_i g_it and sum_i g_it xs_it.
@@ -390,11 +576,17 @@ size_t backward_scratch_bytes(int n, int d, int k) {
     const size_t nn = static_cast<size_t>(n);
     const size_t blocks = (nn + kBlock - 1) / kBlock;
     size_t b = 0;
-    b += 7 * nn * sizeof(double);               // m, lrd, S, a, gm, gkd + slack
+    b += 6 * nn * sizeof(double);               // m, lrd, S, a, gm, gkd
     b += nn * d * sizeof(double);               // gradient in FP64
+    b += 2 * static_cast<size_t>(d) * sizeof(double);  // z-score sums
     b += (3 * nn + blocks + 8) * sizeof(int);   // count, offset, cursor, block totals
-    b += nn * k * sizeof(int);                  // reverse lists
-    b += 16 * 256;                              // alignment
+    b += nn * k * sizeof(int);                  // reverse lists (then their queries)
+    b += nn * k * sizeof(double);               // coefficients of the reverse terms
+    const size_t chunks = nn * k / kChunk + 1;
+    b += nn * sizeof(int);                      // chunk starts
+    b += chunks * sizeof(int);                  // chunk owners
+    b += chunks * d * sizeof(double);           // partial rows of the extra chunks
+    b += 20 * 256;                              // alignment of the 17 pieces
     return b;
 }
 
@@ -424,6 +616,12 @@ void lof_backward(const float* keyrows, int n, int d, int k, const int* nbr_idx,
     int* cursor = static_cast<int*>(take(nn * sizeof(int)));
     int* totals = static_cast<int*>(take((blocks + 8) * sizeof(int)));
     int* rev = static_cast<int*>(take(nk * sizeof(int)));
+    double* coef = static_cast<double*>(take(nk * sizeof(double)));
+    const size_t chunks = nk / kChunk + 1;
+    int* chunk_start = static_cast<int*>(take(nn * sizeof(int)));
+    int* owner = static_cast<int*>(take(chunks * sizeof(int)));
+    double* part = static_cast<double*>(take(chunks * d * sizeof(double)));
+    int* extra = cursor;  // free once the reverse lists are filled
 
     ProfileScope prof("lof_backward", s);
     point_m_kernel<<<grid_for(nn), kBlock, 0, s>>>(nbr_idx, nbr_d2, kdist2, n, k, m, lrd);
@@ -435,25 +633,35 @@ void lof_backward(const float* keyrows, int n, int d, int k, const int* nbr_idx,
     scan_totals_kernel<<<1, kBlock, 0, s>>>(totals, static_cast<int>(blocks));
     scan_add_kernel<<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(offset, n, totals);
     rev_fill_kernel<<<grid_for(nk), kBlock, 0, s>>>(nbr_idx, nk, offset, cursor, rev);
-    rev_sort_kernel<<<grid_for(nn), kBlock, 0, s>>>(rev, offset, count, n);
+    rev_sort_kernel<<<grid_for(nn), kSortThreads, 0, s>>>(rev, offset, count, n);
     grad_m_kernel<<<grid_for(nn), kBlock, 0, s>>>(rev, offset, count, a, S, lrd, grad_scores, n, gm);
     grad_kd_kernel<<<grid_for(nn), kBlock, 0, s>>>(rev, offset, count, nbr_d2, kdist2, gm, n, k, gkd);
-    if (d <= 4) {
-        launch_grad_x<4, 1>(keyrows, d, nbr_idx, nbr_d2, kdist2, rev, offset, count, gm, gkd, n, k, gx, s);
-    } else if (d <= 8) {
-        launch_grad_x<8, 1>(keyrows, d, nbr_idx, nbr_d2, kdist2, rev, offset, count, gm, gkd, n, k, gx, s);
-    } else if (d <= 16) {
-        launch_grad_x<16, 1>(keyrows, d, nbr_idx, nbr_d2, kdist2, rev, offset, count, gm, gkd, n, k, gx, s);
-    } else if (d <= 32) {
-        launch_grad_x<32, 1>(keyrows, d, nbr_idx, nbr_d2, kdist2, rev, offset, count, gm, gkd, n, k, gx, s);
-    } else if (d <= 64) {
-        launch_grad_x<64, 1>(keyrows, d, nbr_idx, nbr_d2, kdist2, rev, offset, count, gm, gkd, n, k, gx, s);
-    } else if (d <= 128) {
-        launch_grad_x<64, 2>(keyrows, d, nbr_idx, nbr_d2, kdist2, rev, offset, count, gm, gkd, n, k, gx, s);
-    } else {
-        launch_grad_x<64, kMaxBackwardDims / 64>(keyrows, d, nbr_idx, nbr_d2, kdist2, rev, offset, count, gm,
-                                                 gkd, n, k, gx, s);
-    }
+    rev_coef_kernel<<<grid_for(nk), kBlock, 0, s>>>(rev, nk, nbr_idx, nbr_d2, kdist2, gm, gkd, n, k, coef);
+    chunk_count_kernel<<<grid_for(nn), kBlock, 0, s>>>(count, n, extra);
+    scan_blocks_kernel<<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(extra, n, chunk_start, totals);
+    scan_totals_kernel<<<1, kBlock, 0, s>>>(totals, static_cast<int>(blocks));
+    scan_add_kernel<<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(chunk_start, n, totals);
+    chunk_owner_kernel<<<grid_for(nn), kBlock, 0, s>>>(extra, chunk_start, n, owner);
+    // G lanes per point with F <= 4 features each: few lanes repeat the per-term
+    // work (group sum, square root, division) while rows are still read in
+    // G-wide runs.
+    auto grad_x = [&](auto g, auto f) {
+        launch_grad_x<decltype(g)::value, decltype(f)::value>(keyrows, d, nbr_idx, nbr_d2, kdist2, rev, coef,
+                                                              offset, count, gm, gkd, extra, chunk_start,
+                                                              owner, n, k, gx, part, s);
+    };
+    using I = std::integral_constant<int, 1>;
+    using II = std::integral_constant<int, 2>;
+    using IV = std::integral_constant<int, 4>;
+    if (d <= 4) grad_x(std::integral_constant<int, 4>{}, I{});
+    else if (d <= 8) grad_x(std::integral_constant<int, 8>{}, I{});
+    else if (d <= 16) grad_x(std::integral_constant<int, 8>{}, II{});
+    else if (d <= 32) grad_x(std::integral_constant<int, 8>{}, IV{});
+    else if (d <= 64) grad_x(std::integral_constant<int, 16>{}, IV{});
+    else if (d <= 128) grad_x(std::integral_constant<int, 32>{}, IV{});
+    else grad_x(std::integral_constant<int, 64>{}, IV{});
+    static_assert(kMaxBackwardDims <= 64 * 4, "the widest grad_x shape covers 256 features");
+    chunk_finish_kernel<<<grid_for(nn * d), kBlock, 0, s>>>(extra, chunk_start, part, n, d, gx);
     if (inv_std) {
         zscore_sums_kernel<<<d, kBlock, 0, s>>>(gx, keyrows, n, d, zsums);
         zscore_apply_kernel<<<grid_for(nn * d), kBlock, 0, s>>>(gx, keyrows, zsums, inv_std, scaled, n, d, grad_points);
