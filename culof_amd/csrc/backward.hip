@@ -224,49 +224,115 @@ rev_sort_kernel(int* __restrict__ rev, const int* __restrict__ offset,
 // its term.
 constexpr int kBatch = 8;
 
-// g_m_p = w_p S_p - lrd_p^2 sum_{i : p in N(i)} a_i
+// A point in many lists would keep its lane or group busy long after the
+// rest, so every pass over the reverse lists cuts them into chunks of kChunk
+// entries: work item t < n takes the first chunk of point t's list, work item
+// n + x the extra chunk x (chunk_start[p] + c - 1 is the c-th chunk of p for
+// c >= 1), whose partial result a finishing pass adds to the point's in chunk
+// order.
+constexpr int kChunk = 64;
+
+// extra[p] = chunks of p's reverse list after the first
 __global__ void __launch_bounds__(kBlock)
-grad_m_kernel(const int* __restrict__ rev, const int* __restrict__ offset,
-              const int* __restrict__ count, const double* __restrict__ a,
-              const double* __restrict__ S, const double* __restrict__ lrd,
-              const float* __restrict__ w, int n, double* __restrict__ gm) {
+chunk_count_kernel(const int* __restrict__ count, int n, int* __restrict__ extra) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) extra[p] = count[p] > 0 ? (count[p] - 1) / kChunk : 0;
+}
+
+// owner[chunk_start[p] + t] = p for t < extra[p]
+__global__ void __launch_bounds__(kBlock)
+chunk_owner_kernel(const int* __restrict__ extra, const int* __restrict__ chunk_start, int n,
+                   int* __restrict__ owner) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    double g_lrd = 0.0;
+    for (int t = 0; t < extra[p]; ++t) owner[chunk_start[p] + t] = p;
+}
+
+// Point and chunk of work item t; false past the last extra chunk.
+__device__ bool chunk_of(int t, int n, const int* __restrict__ extra,
+                         const int* __restrict__ chunk_start, const int* __restrict__ owner,
+                         int& p, int& chunk) {
+    if (t < n) {
+        p = t;
+        chunk = 0;
+        return true;
+    }
+    const int x = t - n;
+    if (x >= chunk_start[n - 1] + extra[n - 1]) return false;
+    p = owner[x];
+    chunk = x - chunk_start[p] + 1;
+    return true;
+}
+
+// Work items: n points plus at most n k / kChunk extra chunks.
+size_t chunk_items(size_t n, int k) { return n + n * k / kChunk; }
+
+// sum_{i : p in N(i)} a_i over one chunk of p's list; the first chunk's sum
+// goes to gm[p], an extra chunk's to part[x].
+__global__ void __launch_bounds__(kBlock)
+grad_m_chunk_kernel(const int* __restrict__ rev, const int* __restrict__ offset,
+                    const int* __restrict__ count, const double* __restrict__ a,
+                    const int* __restrict__ extra, const int* __restrict__ chunk_start,
+                    const int* __restrict__ owner, int n, double* __restrict__ gm,
+                    double* __restrict__ part) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int p, chunk;
+    if (!chunk_of(t, n, extra, chunk_start, owner, p, chunk)) return;
+    double sum = 0.0;
     const int* seg = rev + offset[p];
-    const int len = count[p];
-    for (int r0 = 0; r0 < len; r0 += kBatch) {
+    const int lo = chunk * kChunk;
+    const int hi = min(lo + kChunk, count[p]);
+    for (int r0 = lo; r0 < hi; r0 += kBatch) {
         int e[kBatch];
         double v[kBatch];
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) e[u] = seg[min(r0 + u, len - 1)];
+        for (int u = 0; u < kBatch; ++u) e[u] = seg[min(r0 + u, hi - 1)];
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) v[u] = a[e[u] % n];
 #pragma unroll
         for (int u = 0; u < kBatch; ++u)
-            if (r0 + u < len) g_lrd += v[u];
+            if (r0 + u < hi) sum += v[u];
     }
+    if (chunk == 0) gm[p] = sum;
+    else part[t - n] = sum;
+}
+
+// g_m_p = w_p S_p - lrd_p^2 g_lrd_p, g_lrd_p = the chunk sums in chunk order
+__global__ void __launch_bounds__(kBlock)
+grad_m_finish_kernel(const int* __restrict__ extra, const int* __restrict__ chunk_start,
+                     const double* __restrict__ part, const double* __restrict__ S,
+                     const double* __restrict__ lrd, const float* __restrict__ w, int n,
+                     double* __restrict__ gm) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double g_lrd = gm[p];
+    for (int t = 0; t < extra[p]; ++t) g_lrd += part[chunk_start[p] + t];
     gm[p] = static_cast<double>(w[p]) * S[p] - g_lrd * lrd[p] * lrd[p];
 }
 
-// g_kd_p = sum over i with p in N(i) whose reach took the kd branch of g_m_i / k
+// g_kd_p = sum over i with p in N(i) whose reach took the kd branch of g_m_i / k,
+// one chunk of p's list: the first chunk's sum to gkd[p], an extra one's to part[x]
 __global__ void __launch_bounds__(kBlock)
-grad_kd_kernel(const int* __restrict__ rev, const int* __restrict__ offset,
-               const int* __restrict__ count, const float* __restrict__ d2,
-               const float* __restrict__ kd2, const double* __restrict__ gm, int n, int k,
-               double* __restrict__ gkd) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    double s = 0.0;
+grad_kd_chunk_kernel(const int* __restrict__ rev, const int* __restrict__ offset,
+                     const int* __restrict__ count, const float* __restrict__ d2,
+                     const float* __restrict__ kd2, const double* __restrict__ gm,
+                     const int* __restrict__ extra, const int* __restrict__ chunk_start,
+                     const int* __restrict__ owner, int n, int k, double* __restrict__ gkd,
+                     double* __restrict__ part) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int p, chunk;
+    if (!chunk_of(t, n, extra, chunk_start, owner, p, chunk)) return;
+    double sum = 0.0;
     const float kd2p = kd2[p];
     const int* seg = rev + offset[p];
-    const int len = count[p];
-    for (int r0 = 0; r0 < len; r0 += kBatch) {
+    const int lo = chunk * kChunk;
+    const int hi = min(lo + kChunk, count[p]);
+    for (int r0 = lo; r0 < hi; r0 += kBatch) {
         int e[kBatch];
         float de[kBatch];
         double g[kBatch];
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) e[u] = seg[min(r0 + u, len - 1)];
+        for (int u = 0; u < kBatch; ++u) e[u] = seg[min(r0 + u, hi - 1)];
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
             de[u] = d2[e[u]];
@@ -274,8 +340,19 @@ grad_kd_kernel(const int* __restrict__ rev, const int* __restrict__ offset,
         }
 #pragma unroll
         for (int u = 0; u < kBatch; ++u)
-            if (r0 + u < len && kd2p > de[u]) s += g[u] / k;
+            if (r0 + u < hi && kd2p > de[u]) sum += g[u] / k;
     }
+    if (chunk == 0) gkd[p] = sum;
+    else part[t - n] = sum;
+}
+
+__global__ void __launch_bounds__(kBlock)
+grad_kd_finish_kernel(const int* __restrict__ extra, const int* __restrict__ chunk_start,
+                      const double* __restrict__ part, int n, double* __restrict__ gkd) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double s = gkd[p];
+    for (int t = 0; t < extra[p]; ++t) s += part[chunk_start[p] + t];
     gkd[p] = s;
 }
 
@@ -381,31 +458,9 @@ struct GroupRow {
 
 // Every term a point owns or receives (own list, own k-distance, reverse
 // list) accumulated into its FP64 gradient row; O(n k d) in total. Terms are
-// fetched kBatch at a time and added in list order.
-//
-// A point in many lists would leave its group running long after the rest, so
-// reverse lists are cut into chunks of kChunk entries: group p (p < n) takes
-// point p's own terms and the first chunk of its list; every further chunk is
-// extra chunk x, taken by group n + x, which writes a partial row that
-// chunk_finish_kernel adds to the point's row in chunk order.
-constexpr int kChunk = 64;
-
-// extra[p] = chunks of p's reverse list after the first
-__global__ void __launch_bounds__(kBlock)
-chunk_count_kernel(const int* __restrict__ count, int n, int* __restrict__ extra) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) extra[p] = count[p] > 0 ? (count[p] - 1) / kChunk : 0;
-}
-
-// owner[chunk_start[p] + t] = p for t < extra[p]
-__global__ void __launch_bounds__(kBlock)
-chunk_owner_kernel(const int* __restrict__ extra, const int* __restrict__ chunk_start, int n,
-                   int* __restrict__ owner) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    for (int t = 0; t < extra[p]; ++t) owner[chunk_start[p] + t] = p;
-}
-
+// fetched kBatch at a time and added in list order. Group p (p < n) takes
+// point p's own terms and the first chunk of its reverse list, group n + x
+// extra chunk x, whose partial row chunk_finish_kernel adds to the point's.
 template <int G, int F>
 __global__ void __launch_bounds__(kBlock)
 grad_x_kernel(const float* __restrict__ xs, int d, const int* __restrict__ nbr,
@@ -417,14 +472,8 @@ grad_x_kernel(const float* __restrict__ xs, int d, const int* __restrict__ nbr,
               const int* __restrict__ owner, int n, int k, double* __restrict__ gx,
               double* __restrict__ part) {
     const int g = (blockIdx.x * kBlock + threadIdx.x) / G;  // whole groups take one branch
-    int p = g;
-    int chunk = 0;
-    if (g >= n) {
-        const int x = g - n;
-        if (x >= chunk_start[n - 1] + extra[n - 1]) return;
-        p = owner[x];
-        chunk = x - chunk_start[p] + 1;
-    }
+    int p, chunk;
+    if (!chunk_of(g, n, extra, chunk_start, owner, p, chunk)) return;
     const GroupRow<G, F> row{static_cast<int>(threadIdx.x % G), d};
     double xp[F], acc[F];
     row.load(xs + static_cast<size_t>(p) * d, xp);
@@ -509,9 +558,7 @@ void launch_grad_x(const float* xs, int d, const int* nbr, const float* d2, cons
                    const int* src, const double* coef, const int* offset, const int* count,
                    const double* gm, const double* gkd, const int* extra, const int* chunk_start,
                    const int* owner, int n, int k, double* gx, double* part, hipStream_t s) {
-    // n point groups plus at most n k / kChunk extra chunks
-    const size_t groups = static_cast<size_t>(n) + static_cast<size_t>(n) * k / kChunk;
-    const size_t lanes = groups * G;
+    const size_t lanes = chunk_items(static_cast<size_t>(n), k) * G;
     grad_x_kernel<G, F><<<static_cast<unsigned>((lanes + kBlock - 1) / kBlock), kBlock, 0, s>>>(
         xs, d, nbr, d2, kd2, src, coef, offset, count, gm, gkd, extra, chunk_start, owner, n, k, gx, part);
 }
@@ -634,14 +681,18 @@ void lof_backward(const float* keyrows, int n, int d, int k, const int* nbr_idx,
     scan_add_kernel<<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(offset, n, totals);
     rev_fill_kernel<<<grid_for(nk), kBlock, 0, s>>>(nbr_idx, nk, offset, cursor, rev);
     rev_sort_kernel<<<grid_for(nn), kSortThreads, 0, s>>>(rev, offset, count, n);
-    grad_m_kernel<<<grid_for(nn), kBlock, 0, s>>>(rev, offset, count, a, S, lrd, grad_scores, n, gm);
-    grad_kd_kernel<<<grid_for(nn), kBlock, 0, s>>>(rev, offset, count, nbr_d2, kdist2, gm, n, k, gkd);
-    rev_coef_kernel<<<grid_for(nk), kBlock, 0, s>>>(rev, nk, nbr_idx, nbr_d2, kdist2, gm, gkd, n, k, coef);
     chunk_count_kernel<<<grid_for(nn), kBlock, 0, s>>>(count, n, extra);
     scan_blocks_kernel<<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(extra, n, chunk_start, totals);
     scan_totals_kernel<<<1, kBlock, 0, s>>>(totals, static_cast<int>(blocks));
     scan_add_kernel<<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(chunk_start, n, totals);
     chunk_owner_kernel<<<grid_for(nn), kBlock, 0, s>>>(extra, chunk_start, n, owner);
+    const unsigned items = grid_for(chunk_items(nn, k));
+    grad_m_chunk_kernel<<<items, kBlock, 0, s>>>(rev, offset, count, a, extra, chunk_start, owner, n, gm, part);
+    grad_m_finish_kernel<<<grid_for(nn), kBlock, 0, s>>>(extra, chunk_start, part, S, lrd, grad_scores, n, gm);
+    grad_kd_chunk_kernel<<<items, kBlock, 0, s>>>(rev, offset, count, nbr_d2, kdist2, gm, extra, chunk_start,
+                                                  owner, n, k, gkd, part);
+    grad_kd_finish_kernel<<<grid_for(nn), kBlock, 0, s>>>(extra, chunk_start, part, n, gkd);
+    rev_coef_kernel<<<grid_for(nk), kBlock, 0, s>>>(rev, nk, nbr_idx, nbr_d2, kdist2, gm, gkd, n, k, coef);
     // G lanes per point with F <= 4 features each: few lanes repeat the per-term
     // work (group sum, square root, division) while rows are still read in
     // G-wide runs.
