@@ -194,6 +194,35 @@ def test_backward_is_deterministic_and_form_independent():
     np.testing.assert_array_equal(g_bf16, first)
 
 
+def test_hub_points_and_long_reverse_lists():
+    """Hubs (points in hundreds of kNN lists) at k = 40: reverse lists longer
+    than 64 entries are summed in several chunks, and most 256-point blocks
+    hold more than 8192 reverse entries, so they sort each list in global
+    memory rather than in LDS. Both paths against FP64 autograd, and a repeat
+    gives the same bits."""
+    torch = _torch()
+    from culof_amd.autograd import backward_points, forward_state
+
+    n, d, k = 3000, 32, 40
+    X = make_data(n, d, 21, n // 100)
+    w = np.random.default_rng(21).standard_normal(n).astype(np.float32)
+    idx, _, _, _ = _state(X, k)
+    indeg = np.bincount(idx.ravel(), minlength=n)
+    spans = np.add.reduceat(indeg, np.arange(0, n, 256))
+    assert indeg.max() > 4 * 64, indeg.max()   # a list of five or more chunks
+    assert (indeg > 64).sum() > 100
+    assert (spans > 8192).any() and (spans <= 8192).any(), spans  # both sort paths
+    _, g = _gpu_grad(X, k, w)
+    want, _ = _fixed_reference(X, k, w)
+    _assert_close(g, want, 2e-5, "gradient vs FP64 autograd (hub points)")
+    x = torch.from_numpy(X).cuda()
+    st = forward_state(x, k)
+    wt = torch.from_numpy(w).cuda()
+    first = backward_points(x, st, wt).cpu().numpy()
+    np.testing.assert_array_equal(backward_points(x, st, wt).cpu().numpy(), first)
+    np.testing.assert_array_equal(first, g)
+
+
 def test_duplicates_give_finite_gradients():
     X = make_data(1000, 8, 17, 10)
     X[500:520] = X[100]  # 21 coincident points: zero distances inside every list
