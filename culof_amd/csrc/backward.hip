@@ -221,8 +221,12 @@ rev_sort_kernel(int* __restrict__ rev, const int* __restrict__ offset,
 // memory latency per batch rather than per entry, while the terms are still
 // added one by one in list order. Past the end of a list a batch reads the
 // list's last entry again (the loop only runs for non-empty lists) and drops
-// its term.
-constexpr int kBatch = 8;
+// its term. The sums of g_m and g_kd take 8 entries per batch; grad_x takes
+// 2, as with its branch-free terms the waves per SIMD matter more than the
+// loads in flight per wave (50k x 32: 8 entries, 154 VGPRs, 0.51 ms for the
+// backward; 4, 94 VGPRs, 0.46 ms; 2, 0.42 ms; 1, 0.43 ms).
+constexpr int kSumBatch = 8;
+constexpr int kBatch = 2;
 
 // A point in many lists would keep its lane or group busy long after the
 // rest, so every pass over the reverse lists cuts them into chunks of kChunk
@@ -282,15 +286,15 @@ grad_m_chunk_kernel(const int* __restrict__ rev, const int* __restrict__ offset,
     const int* seg = rev + offset[p];
     const int lo = chunk * kChunk;
     const int hi = min(lo + kChunk, count[p]);
-    for (int r0 = lo; r0 < hi; r0 += kBatch) {
-        int e[kBatch];
-        double v[kBatch];
+    for (int r0 = lo; r0 < hi; r0 += kSumBatch) {
+        int e[kSumBatch];
+        double v[kSumBatch];
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) e[u] = seg[min(r0 + u, hi - 1)];
+        for (int u = 0; u < kSumBatch; ++u) e[u] = seg[min(r0 + u, hi - 1)];
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) v[u] = a[e[u] % n];
+        for (int u = 0; u < kSumBatch; ++u) v[u] = a[e[u] % n];
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u)
+        for (int u = 0; u < kSumBatch; ++u)
             if (r0 + u < hi) sum += v[u];
     }
     if (chunk == 0) gm[p] = sum;
@@ -327,19 +331,19 @@ grad_kd_chunk_kernel(const int* __restrict__ rev, const int* __restrict__ offset
     const int* seg = rev + offset[p];
     const int lo = chunk * kChunk;
     const int hi = min(lo + kChunk, count[p]);
-    for (int r0 = lo; r0 < hi; r0 += kBatch) {
-        int e[kBatch];
-        float de[kBatch];
-        double g[kBatch];
+    for (int r0 = lo; r0 < hi; r0 += kSumBatch) {
+        int e[kSumBatch];
+        float de[kSumBatch];
+        double g[kSumBatch];
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) e[u] = seg[min(r0 + u, hi - 1)];
+        for (int u = 0; u < kSumBatch; ++u) e[u] = seg[min(r0 + u, hi - 1)];
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) {
+        for (int u = 0; u < kSumBatch; ++u) {
             de[u] = d2[e[u]];
             g[u] = gm[e[u] % n];
         }
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u)
+        for (int u = 0; u < kSumBatch; ++u)
             if (r0 + u < hi && kd2p > de[u]) sum += g[u] / k;
     }
     if (chunk == 0) gkd[p] = sum;
