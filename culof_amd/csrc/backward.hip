@@ -19,7 +19,8 @@
 //
 // Every sum runs in FP64 in a fixed order. Contributions owned by another
 // point come through reverse neighbour lists (CSR): counted with integer
-// atomics, filled, then each segment sorted by (slot, query) linear index, so
+// atomics (each entry keeping the place it took), filled by those places, then
+// each segment sorted by (slot, query) linear index, so
 // the result is bit-identical from run to run with no floating-point atomics.
 // The distances are recomputed from the key rows in FP64 (n k pairs) instead
 // of storing anything n x n. With normalisation the gradient is carried back
@@ -71,10 +72,12 @@ point_s_kernel(const int* __restrict__ nbr, const double* __restrict__ lrd,
 
 // ---- reverse lists ----
 
+// count[p] = entries whose neighbour is p; each entry keeps the place it took
+// in p's list (in whatever order the atomics ran: the lists are sorted later).
 __global__ void __launch_bounds__(kBlock)
-rev_count_kernel(const int* __restrict__ nbr, size_t nk, int* __restrict__ count) {
+rev_count_kernel(const int* __restrict__ nbr, size_t nk, int* __restrict__ count, int* __restrict__ place) {
     const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e < nk) atomicAdd(&count[nbr[e]], 1);
+    if (e < nk) place[e] = atomicAdd(&count[nbr[e]], 1);
 }
 
 // Exclusive scan, three steps: per-block scan + block totals, scan of the
@@ -132,11 +135,10 @@ scan_add_kernel(int* __restrict__ out, int n, const int* __restrict__ totals) {
 
 __global__ void __launch_bounds__(kBlock)
 rev_fill_kernel(const int* __restrict__ nbr, size_t nk, const int* __restrict__ offset,
-                int* __restrict__ cursor, int* __restrict__ rev) {
+                const int* __restrict__ place, int* __restrict__ rev) {
     const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= nk) return;
-    const int p = nbr[e];
-    rev[offset[p] + atomicAdd(&cursor[p], 1)] = static_cast<int>(e);  // e = slot * n + query
+    rev[offset[nbr[e]] + place[e]] = static_cast<int>(e);  // e = slot * n + query
 }
 
 // Heapsort each segment ascending, one thread per point: O(L log L) even for
@@ -630,7 +632,7 @@ size_t backward_scratch_bytes(int n, int d, int k) {
     b += 6 * nn * sizeof(double);               // m, lrd, S, a, gm, gkd
     b += nn * d * sizeof(double);               // gradient in FP64
     b += 2 * static_cast<size_t>(d) * sizeof(double);  // z-score sums
-    b += (3 * nn + blocks + 8) * sizeof(int);   // count, offset, cursor, block totals
+    b += (3 * nn + blocks + 8) * sizeof(int);   // count, offset, extra chunks, block totals
     b += nn * k * sizeof(int);                  // reverse lists (then their queries)
     b += nn * k * sizeof(double);               // coefficients of the reverse terms
     const size_t chunks = nn * k / kChunk + 1;
@@ -664,7 +666,7 @@ void lof_backward(const float* keyrows, int n, int d, int k, const int* nbr_idx,
     double* zsums = static_cast<double*>(take(2 * d * sizeof(double)));
     int* count = static_cast<int*>(take(nn * sizeof(int)));
     int* offset = static_cast<int*>(take(nn * sizeof(int)));
-    int* cursor = static_cast<int*>(take(nn * sizeof(int)));
+    int* extra = static_cast<int*>(take(nn * sizeof(int)));  // chunks per list after the first
     int* totals = static_cast<int*>(take((blocks + 8) * sizeof(int)));
     int* rev = static_cast<int*>(take(nk * sizeof(int)));
     double* coef = static_cast<double*>(take(nk * sizeof(double)));
@@ -672,18 +674,17 @@ void lof_backward(const float* keyrows, int n, int d, int k, const int* nbr_idx,
     int* chunk_start = static_cast<int*>(take(nn * sizeof(int)));
     int* owner = static_cast<int*>(take(chunks * sizeof(int)));
     double* part = static_cast<double*>(take(chunks * d * sizeof(double)));
-    int* extra = cursor;  // free once the reverse lists are filled
 
     ProfileScope prof("lof_backward", s);
     point_m_kernel<<<grid_for(nn), kBlock, 0, s>>>(nbr_idx, nbr_d2, kdist2, n, k, m, lrd);
     point_s_kernel<<<grid_for(nn), kBlock, 0, s>>>(nbr_idx, lrd, m, grad_scores, n, k, S, a);
+    int* place = reinterpret_cast<int*>(coef);  // n k ints, free until rev_coef_kernel
     zero_fill(count, nn * sizeof(int), s);
-    zero_fill(cursor, nn * sizeof(int), s);
-    rev_count_kernel<<<grid_for(nk), kBlock, 0, s>>>(nbr_idx, nk, count);
+    rev_count_kernel<<<grid_for(nk), kBlock, 0, s>>>(nbr_idx, nk, count, place);
     scan_blocks_kernel<<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(count, n, offset, totals);
     scan_totals_kernel<<<1, kBlock, 0, s>>>(totals, static_cast<int>(blocks));
     scan_add_kernel<<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(offset, n, totals);
-    rev_fill_kernel<<<grid_for(nk), kBlock, 0, s>>>(nbr_idx, nk, offset, cursor, rev);
+    rev_fill_kernel<<<grid_for(nk), kBlock, 0, s>>>(nbr_idx, nk, offset, place, rev);
     rev_sort_kernel<<<grid_for(nn), kSortThreads, 0, s>>>(rev, offset, count, n);
     chunk_count_kernel<<<grid_for(nn), kBlock, 0, s>>>(count, n, extra);
     scan_blocks_kernel<<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(extra, n, chunk_start, totals);
