@@ -125,7 +125,7 @@ def test_plain_20k_whole_forward_and_fp64_sklearn():
     np.testing.assert_array_equal(got["idx"].T, want["idx"])
     _same_bits(got["d2"].T, want["d2"], f"{name} nbr_d2")
 
-    # scikit-learn fitted on float64 input: its FP64 path (the north star's comparator)
+    # scikit-learn fitted on float64 input: its FP64 path, the comparator of the accuracy targets
     sk, sk_labels = _sklearn_fp64(X, k, contamination=0.01)
     med, p99, agree = _agreement(got["scores"], sk)
     assert med <= 1e-6 and p99 <= 5e-6 and agree == 1.0, (med, p99, agree)
