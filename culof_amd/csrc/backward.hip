@@ -171,14 +171,15 @@ __device__ void heap_sort(int* a, int len) {
 }
 
 // The kBlock consecutive points of a block own one contiguous CSR range. When
-// it fits (kSortLds entries) it is copied to LDS and every entry finds its
+// it fits (kSortLds entries: 128 KB, an average of 128 entries per list, so
+// any k up to about 100) it is copied to LDS and every entry finds its
 // place in its own segment by counting the smaller entries there (the entries
 // of a segment are distinct): the work is spread over the block's lanes by
 // entry, so a hub point that sits in hundreds of lists costs L^2 / kSortThreads
 // steps per lane instead of a serial sort of L entries by one lane. Ranges that do
 // not fit sort each segment in place in global memory.
 // kSortThreads lanes share the ranking of the kBlock points' entries.
-constexpr int kSortLds = 8192;
+constexpr int kSortLds = 32768;
 constexpr int kSortThreads = 1024;
 
 __global__ void __launch_bounds__(kSortThreads)

@@ -194,16 +194,18 @@ def test_backward_is_deterministic_and_form_independent():
     np.testing.assert_array_equal(g_bf16, first)
 
 
-def test_hub_points_and_long_reverse_lists():
-    """Hubs (points in hundreds of kNN lists) at k = 40: reverse lists longer
-    than 64 entries are summed in several chunks, and most 256-point blocks
-    hold more than 8192 reverse entries, so they sort each list in global
-    memory rather than in LDS. Both paths against FP64 autograd, and a repeat
-    gives the same bits."""
+@pytest.mark.parametrize("k", [40, 140])
+def test_hub_points_and_long_reverse_lists(k):
+    """Hubs (points in hundreds of kNN lists): reverse lists longer than 64
+    entries are summed in several chunks. At k = 40 every 256-point block
+    sorts its lists in LDS (at most 32,768 entries); at k = 140 most blocks
+    hold more and sort each list in global memory. Both against FP64
+    autograd, and a repeat gives the same bits."""
     torch = _torch()
     from culof_amd.autograd import backward_points, forward_state
 
-    n, d, k = 3000, 32, 40
+    n, d = 3000, 32
+    lds_entries = 32768
     X = make_data(n, d, 21, n // 100)
     w = np.random.default_rng(21).standard_normal(n).astype(np.float32)
     idx, _, _, _ = _state(X, k)
@@ -211,10 +213,13 @@ def test_hub_points_and_long_reverse_lists():
     spans = np.add.reduceat(indeg, np.arange(0, n, 256))
     assert indeg.max() > 4 * 64, indeg.max()   # a list of five or more chunks
     assert (indeg > 64).sum() > 100
-    assert (spans > 8192).any() and (spans <= 8192).any(), spans  # both sort paths
+    if k == 40:
+        assert (spans <= lds_entries).all(), spans
+    else:
+        assert (spans > lds_entries).any() and (spans <= lds_entries).any(), spans
     _, g = _gpu_grad(X, k, w)
     want, _ = _fixed_reference(X, k, w)
-    _assert_close(g, want, 2e-5, "gradient vs FP64 autograd (hub points)")
+    _assert_close(g, want, 2e-5, f"gradient vs FP64 autograd (hub points, k={k})")
     x = torch.from_numpy(X).cuda()
     st = forward_state(x, k)
     wt = torch.from_numpy(w).cuda()
