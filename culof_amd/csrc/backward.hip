@@ -170,15 +170,15 @@ __device__ void heap_sort(int* a, int len) {
     }
 }
 
-// The kBlock consecutive points of a block own one contiguous CSR range. When
-// it fits (kSortLds entries: 128 KB, an average of 128 entries per list, so
-// any k up to about 100) it is copied to LDS and every entry finds its
-// place in its own segment by counting the smaller entries there (the entries
-// of a segment are distinct): the work is spread over the block's lanes by
-// entry, so a hub point that sits in hundreds of lists costs L^2 / kSortThreads
-// steps per lane instead of a serial sort of L entries by one lane. Ranges that do
-// not fit sort each segment in place in global memory.
-// kSortThreads lanes share the ranking of the kBlock points' entries.
+// The kBlock consecutive points of a block own one contiguous CSR range, sorted
+// in windows of whole lists that fit in LDS (kSortLds entries, 128 KB: the
+// whole range while lists average up to 128 entries, so any k up to about 100
+// takes one window). A window is copied to LDS and every entry finds its place
+// in its own list by counting the smaller entries there (the entries of a list
+// are distinct); the work is spread over the block's lanes by entry, so a hub
+// point that sits in hundreds of lists costs L^2 / kSortThreads steps per lane
+// instead of a serial sort of L entries by one lane. A single list longer than
+// kSortLds is heap-sorted in place in global memory by one lane.
 constexpr int kSortLds = 32768;
 constexpr int kSortThreads = 1024;
 
@@ -186,36 +186,51 @@ __global__ void __launch_bounds__(kSortThreads)
 rev_sort_kernel(int* __restrict__ rev, const int* __restrict__ offset,
                 const int* __restrict__ count, int n) {
     __shared__ int s_seg[kSortLds];
-    __shared__ int s_off[kBlock + 1];  // segment starts relative to the range, then its length
+    __shared__ int s_off[kBlock + 1];  // list starts relative to the range, then its length
     const int p0 = blockIdx.x * kBlock;
     const int pts = min(kBlock, n - p0);
     const int begin = offset[p0];
     const int span = offset[p0 + pts - 1] + count[p0 + pts - 1] - begin;
-    const int p = p0 + threadIdx.x;
-    if (span > kSortLds) {
-        if (static_cast<int>(threadIdx.x) < pts) heap_sort(rev + offset[p], count[p]);
-        return;
-    }
-    if (static_cast<int>(threadIdx.x) < pts) s_off[threadIdx.x] = offset[p] - begin;
+    if (static_cast<int>(threadIdx.x) < pts) s_off[threadIdx.x] = offset[p0 + threadIdx.x] - begin;
     if (threadIdx.x == 0) s_off[pts] = span;
-    for (int e = threadIdx.x; e < span; e += kSortThreads) s_seg[e] = rev[begin + e];
     __syncthreads();
-    for (int q = threadIdx.x; q < span; q += kSortThreads) {
-        // the segment holding q: the last start <= q (empty segments share a
-        // start with the next one, whose start is then also <= q)
-        int lo = 0, hi = pts;  // s_off[lo] <= q < s_off[hi]
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (s_off[mid] <= q) lo = mid;
-            else hi = mid;
+    // windows [a, b) of whole lists; every lane walks the same windows
+    for (int a = 0; a < pts;) {
+        const int w0 = s_off[a];
+        if (s_off[a + 1] - w0 > kSortLds) {  // one list longer than LDS holds
+            if (threadIdx.x == 0) heap_sort(rev + begin + w0, s_off[a + 1] - w0);
+            ++a;
+            continue;
         }
-        const int seg_lo = s_off[lo];
-        const int seg_hi = s_off[lo + 1];
-        const int v = s_seg[q];
-        int rank = 0;
+        int lo = a + 1, hi = pts;  // the last b in [a + 1, pts] with s_off[b] - w0 <= kSortLds
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] - w0 <= kSortLds) lo = mid;
+            else hi = mid - 1;
+        }
+        const int b = lo;
+        const int len = s_off[b] - w0;
+        for (int e = threadIdx.x; e < len; e += kSortThreads) s_seg[e] = rev[begin + w0 + e];
+        __syncthreads();
+        for (int q = threadIdx.x; q < len; q += kSortThreads) {
+            // the list holding entry w0 + q: the last start <= it (empty lists
+            // share a start with the next one, whose start is then also <= it)
+            int l = a, h = b;  // s_off[l] <= w0 + q < s_off[h]
+            while (h - l > 1) {
+                const int mid = (l + h) >> 1;
+                if (s_off[mid] <= w0 + q) l = mid;
+                else h = mid;
+            }
+            const int seg_lo = s_off[l] - w0;
+            const int seg_hi = s_off[l + 1] - w0;
+            const int v = s_seg[q];
+            int rank = 0;
 #pragma unroll 4
-        for (int r = seg_lo; r < seg_hi; ++r) rank += s_seg[r] < v ? 1 : 0;
-        rev[begin + seg_lo + rank] = v;
+            for (int r = seg_lo; r < seg_hi; ++r) rank += s_seg[r] < v ? 1 : 0;
+            rev[begin + w0 + seg_lo + rank] = v;
+        }
+        __syncthreads();  // the next window reuses s_seg
+        a = b;
     }
 }
 
