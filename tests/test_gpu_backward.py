@@ -198,9 +198,9 @@ def test_backward_is_deterministic_and_form_independent():
 def test_hub_points_and_long_reverse_lists(k):
     """Hubs (points in hundreds of kNN lists): reverse lists longer than 64
     entries are summed in several chunks. At k = 40 every 256-point block
-    sorts its lists in LDS (at most 32,768 entries); at k = 140 most blocks
-    hold more and sort each list in global memory. Both against FP64
-    autograd, and a repeat gives the same bits."""
+    sorts its lists in one LDS window (at most 32,768 entries); at k = 140
+    most blocks hold more and sort them in several windows. Both against
+    FP64 autograd, and a repeat gives the same bits."""
     torch = _torch()
     from culof_amd.autograd import backward_points, forward_state
 
